@@ -1,0 +1,261 @@
+"""fdt_amd -- MI355X xfibres engine (Bayesian ball-and-sticks estimation).
+
+Python view of the C ABI in include/fdt_xfibres.h (libfdt_xfibres.so).  The
+engine is the HIP library; this module only moves numpy arrays across the
+boundary.  Loading fails loudly when the library has not been built: there is
+no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libfdt_xfibres.so")
+CLI_PATH = os.path.join(HERE, "bin", "xfibres_gpu")
+
+LAYOUT_VOX_MAJOR, LAYOUT_VOL_MAJOR = 0, 1
+KIND_SINGLE, KIND_SINGLE_C, KIND_MULTI = 0, 1, 2
+
+EXPORTS = ["fdt_default_opts", "fdt_create", "fdt_destroy", "fdt_last_error", "fdt_set_acquisition",
+           "fdt_load_data", "fdt_num_params", "fdt_fit", "fdt_lm_stage", "fdt_lm_start", "fdt_get_params",
+           "fdt_set_params", "fdt_mcmc", "fdt_finish", "fdt_get_outputs", "fdt_run", "fdt_sync",
+           "fdt_get_timing", "fdt_set_mcmc_tile"]
+
+
+class Opts(C.Structure):
+    _fields_ = [
+        ("nfibres", C.c_int), ("model", C.c_int), ("fudge", C.c_float), ("njumps", C.c_int), ("nburn", C.c_int),
+        ("sampleevery", C.c_int), ("updateproposalevery", C.c_int), ("seed", C.c_uint64), ("no_ard", C.c_int),
+        ("all_ard", C.c_int), ("nonlin", C.c_int), ("cnonlin", C.c_int), ("rician", C.c_int), ("f0", C.c_int),
+        ("ardf0", C.c_int), ("R_prior_mean", C.c_float), ("R_prior_std", C.c_float), ("R_prior_fudge", C.c_float),
+        ("lm_fp64", C.c_int), ("reserved", C.c_int * 7),
+    ]
+
+
+_FP = C.POINTER(C.c_float)
+_OUT_FIELDS = ["th", "ph", "f", "mean_S0", "mean_d", "mean_dstd", "mean_R", "mean_f0", "mean_tau", "mean_f", "dyads",
+               "disp", "S0", "d", "dstd", "R", "f0", "tau"]
+
+
+class Outputs(C.Structure):
+    _fields_ = [(n, _FP) for n in _OUT_FIELDS]
+
+
+class Timing(C.Structure):
+    _fields_ = [("ms_fit", C.c_float), ("ms_mcmc", C.c_float), ("ms_finish", C.c_float), ("mcmc_vpl", C.c_int),
+                ("mcmc_lanes", C.c_int)]
+
+
+_lib = None
+
+
+def build(jobs: int = 8) -> None:
+    """Compile the HIP library and the xfibres_gpu driver for gfx950 (in-tree)."""
+    subprocess.run(["make", "-s", f"-j{jobs}", "-C", HERE, "all"], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built: run fdt_amd.build() (no CPU fallback exists)")
+        L = C.CDLL(LIB_PATH)
+        vp, ip = C.c_void_p, C.POINTER(C.c_int)
+        L.fdt_default_opts.argtypes = [C.POINTER(Opts)]
+        L.fdt_default_opts.restype = None
+        L.fdt_create.argtypes = [C.c_int, C.POINTER(Opts)]
+        L.fdt_create.restype = vp
+        L.fdt_destroy.argtypes = [vp]
+        L.fdt_destroy.restype = None
+        L.fdt_last_error.argtypes = [vp]
+        L.fdt_last_error.restype = C.c_char_p
+        L.fdt_set_acquisition.argtypes = [vp, C.c_int, _FP, _FP]
+        L.fdt_load_data.argtypes = [vp, _FP, C.c_int, C.c_int]
+        L.fdt_num_params.argtypes = [vp]
+        L.fdt_fit.argtypes = [vp]
+        L.fdt_lm_stage.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float, _FP, _FP, ip]
+        L.fdt_lm_start.argtypes = [vp, C.c_int, C.c_int, C.c_int, _FP]
+        L.fdt_get_params.argtypes = [vp, _FP, _FP]
+        L.fdt_set_params.argtypes = [vp, _FP, _FP]
+        L.fdt_mcmc.argtypes = [vp, C.c_uint64]
+        L.fdt_finish.argtypes = [vp]
+        L.fdt_get_outputs.argtypes = [vp, C.POINTER(Outputs)]
+        L.fdt_run.argtypes = [vp, C.c_uint64]
+        L.fdt_sync.argtypes = [vp]
+        L.fdt_get_timing.argtypes = [vp, C.POINTER(Timing)]
+        L.fdt_set_mcmc_tile.argtypes = [vp, C.c_int]
+        _lib = L
+    return _lib
+
+
+def default_opts(**kw) -> Opts:
+    o = Opts()
+    lib().fdt_default_opts(C.byref(o))
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+def _f32(a):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    return a, a.ctypes.data_as(_FP)
+
+
+class FdtError(RuntimeError):
+    pass
+
+
+class Xfibres:
+    """One engine context on one GPU (mirrors the reference's per-part
+    xfibres_gpu() call, CUDA/xfibres_gpu.cu:29-146, split into phases)."""
+
+    def __init__(self, opts: Opts | None = None, device: int = 0, **kw):
+        self.L = lib()
+        self.o = opts if opts is not None else default_opts(**kw)
+        self.h = self.L.fdt_create(device, C.byref(self.o))
+        if not self.h:
+            raise FdtError("fdt_create failed (bad options or no HIP device)")
+        self.N = 0
+        self.nvox = 0
+
+    def close(self):
+        if self.h:
+            self.L.fdt_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc):
+        if rc != 0:
+            raise FdtError(f"fdt error {rc}: {self.L.fdt_last_error(self.h).decode()}")
+
+    @property
+    def K(self):
+        return self.o.nfibres
+
+    @property
+    def nsamples(self):
+        return self.o.njumps // self.o.sampleevery
+
+    def set_acquisition(self, bvals, bvecs):
+        b, bp = _f32(bvals)
+        g, gp = _f32(np.asarray(bvecs).reshape(3, -1))
+        self._chk(self.L.fdt_set_acquisition(self.h, b.size, bp, gp))
+        self.N = b.size
+
+    def load_data(self, data, layout=LAYOUT_VOX_MAJOR):
+        d, dp = _f32(data)
+        nvox = d.shape[0] if layout == LAYOUT_VOX_MAJOR else d.shape[1]
+        self._chk(self.L.fdt_load_data(self.h, dp, layout, nvox))
+        self.nvox = nvox
+
+    @property
+    def num_params(self):
+        return self.L.fdt_num_params(self.h)
+
+    def fit(self):
+        self._chk(self.L.fdt_fit(self.h))
+
+    def lm_stage(self, kind, K, f0, G, R, start):
+        np_ = (3 if kind == KIND_MULTI else 2) + 3 * K + (1 if f0 else 0)
+        s, sp = _f32(np.asarray(start).reshape(self.nvox, np_))
+        out = np.zeros((self.nvox, np_), np.float32)
+        it = np.zeros(self.nvox, np.int32)
+        self._chk(self.L.fdt_lm_stage(self.h, kind, K, int(f0), G, float(R), sp, out.ctypes.data_as(_FP),
+                                      it.ctypes.data_as(C.POINTER(C.c_int))))
+        return out, it
+
+    def lm_start(self, kind, K, f0):
+        np_ = 2 + 3 * K + (1 if f0 else 0)
+        out = np.zeros((self.nvox, np_), np.float32)
+        self._chk(self.L.fdt_lm_start(self.h, kind, K, int(f0), out.ctypes.data_as(_FP)))
+        return out
+
+    def get_params(self):
+        p = np.zeros((self.nvox, self.num_params), np.float32)
+        t = np.zeros(self.nvox, np.float32)
+        self._chk(self.L.fdt_get_params(self.h, p.ctypes.data_as(_FP), t.ctypes.data_as(_FP)))
+        return p, t
+
+    def set_params(self, params, tau=None):
+        p, pp = _f32(params)
+        tp = _f32(tau)[1] if tau is not None else _FP()
+        self._chk(self.L.fdt_set_params(self.h, pp, tp))
+
+    def mcmc(self, vox_offset=0):
+        self._chk(self.L.fdt_mcmc(self.h, vox_offset))
+
+    def finish(self):
+        self._chk(self.L.fdt_finish(self.h))
+
+    def run(self, vox_offset=0):
+        self._chk(self.L.fdt_run(self.h, vox_offset))
+
+    def sync(self):
+        self._chk(self.L.fdt_sync(self.h))
+
+    def timing(self):
+        t = Timing()
+        self._chk(self.L.fdt_get_timing(self.h, C.byref(t)))
+        return dict(ms_fit=t.ms_fit, ms_mcmc=t.ms_mcmc, ms_finish=t.ms_finish, vpl=t.mcmc_vpl, lanes=t.mcmc_lanes)
+
+    def set_mcmc_tile(self, vpl):
+        self._chk(self.L.fdt_set_mcmc_tile(self.h, vpl))
+
+    def outputs(self, raw_scalars=True):
+        K, ns, nv = self.K, self.nsamples, self.nvox
+        o = self.o
+        arr = dict(th=np.zeros((K, ns, nv), np.float32), ph=np.zeros((K, ns, nv), np.float32),
+                   f=np.zeros((K, ns, nv), np.float32), mean_S0=np.zeros(nv, np.float32),
+                   mean_d=np.zeros(nv, np.float32), mean_f=np.zeros((K, nv), np.float32),
+                   dyads=np.zeros((K, 3, nv), np.float32), disp=np.zeros((K, nv), np.float32))
+        if o.model >= 2:
+            arr["mean_dstd"] = np.zeros(nv, np.float32)
+        if o.model == 3:
+            arr["mean_R"] = np.zeros(nv, np.float32)
+        if o.f0:
+            arr["mean_f0"] = np.zeros(nv, np.float32)
+        if o.rician:
+            arr["mean_tau"] = np.zeros(nv, np.float32)
+        if raw_scalars:
+            arr["S0"] = np.zeros((ns, nv), np.float32)
+            arr["d"] = np.zeros((ns, nv), np.float32)
+            if o.model >= 2:
+                arr["dstd"] = np.zeros((ns, nv), np.float32)
+            if o.model == 3:
+                arr["R"] = np.zeros((ns, nv), np.float32)
+            if o.f0:
+                arr["f0"] = np.zeros((ns, nv), np.float32)
+            if o.rician:
+                arr["tau"] = np.zeros((ns, nv), np.float32)
+        out = Outputs()
+        for n in _OUT_FIELDS:
+            setattr(out, n, arr[n].ctypes.data_as(_FP) if n in arr else _FP())
+        self._chk(self.L.fdt_get_outputs(self.h, C.byref(out)))
+        return arr
+
+
+def run_xfibres(bvals, bvecs, data, seed=8665904, vox_offset=0, layout=LAYOUT_VOX_MAJOR, **opts):
+    """Whole pipeline on one GPU: LM init -> MCMC -> finish; returns outputs and params."""
+    x = Xfibres(seed=seed, **opts)
+    try:
+        x.set_acquisition(bvals, bvecs)
+        x.load_data(data, layout)
+        x.fit()
+        params, tau = x.get_params()
+        x.mcmc(vox_offset)
+        x.finish()
+        out = x.outputs()
+        out["params"] = params
+        out["tau_init"] = tau
+        return out
+    finally:
+        x.close()

@@ -1,0 +1,139 @@
+// fdt_device.h -- device helpers shared by the xfibres kernels (gfx950, wave64).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define FDT_DEV __device__ __forceinline__
+
+namespace fdt {
+
+// diffmodels.h:30-31 / CUDA/diffmodels_utils.h:13-16 (two_pi is 2/pi there)
+constexpr float TWO_PI_F = 0.636619772f;
+constexpr double TWO_PI_D = 0.636619772;
+constexpr float FSMALL_F = 0.001f;
+constexpr float UPPERDIFF_F = 0.005f;  // fibre.h:24
+constexpr float MAXPROP_F = 1e10f;     // fibre.h:19
+
+// ----------------------------------------------------------------- lanes
+// All-reduce inside a 16-lane DPP row: quad_perm xor1, quad_perm xor2,
+// row_half_mirror, row_mirror.  Every lane of the row ends with the same sum
+// (each step adds a partner value, and float addition is commutative).
+template <int CTRL>
+FDT_DEV float dpp(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, 0xF, 0xF, false));
+}
+FDT_DEV float row_sum16(float v) {
+  v += dpp<0xB1>(v);   // quad_perm [1,0,3,2]
+  v += dpp<0x4E>(v);   // quad_perm [2,3,0,1]
+  v += dpp<0x141>(v);  // row_half_mirror
+  v += dpp<0x140>(v);  // row_mirror
+  return v;
+}
+FDT_DEV float row_max16(float v) {
+  v = fmaxf(v, dpp<0xB1>(v));
+  v = fmaxf(v, dpp<0x4E>(v));
+  v = fmaxf(v, dpp<0x141>(v));
+  v = fmaxf(v, dpp<0x140>(v));
+  return v;
+}
+FDT_DEV float row_min16(float v) {
+  v = fminf(v, dpp<0xB1>(v));
+  v = fminf(v, dpp<0x4E>(v));
+  v = fminf(v, dpp<0x141>(v));
+  v = fminf(v, dpp<0x140>(v));
+  return v;
+}
+// Broadcast lane L of each 16-lane row to the whole row (row_newbcast:L).
+template <int L>
+FDT_DEV float row_bcast(float v) {
+  static_assert(L >= 0 && L < 16, "row lane");
+  return dpp<0x150 + L>(v);
+}
+// Whole-wave reductions: the DPP row reduction leaves each 16-lane row's
+// total in all its lanes; the four row totals are read from lanes 0, 16, 32
+// and 48 with v_readlane and combined in a fixed order, so the result is
+// wave-uniform.  (The gfx950 permlane16/32 swaps would keep this on the VALU,
+// but ROCm 7.2 clang folds the two results of a swap whose operands are the
+// same value into one, so they are not used.)
+FDT_DEV float rl(float v, int lane) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), lane));
+}
+FDT_DEV float wave_sum(float v) {
+  v = row_sum16(v);
+  return (rl(v, 0) + rl(v, 16)) + (rl(v, 32) + rl(v, 48));
+}
+FDT_DEV float wave_min(float v) {
+  v = row_min16(v);
+  return fminf(fminf(rl(v, 0), rl(v, 16)), fminf(rl(v, 32), rl(v, 48)));
+}
+
+template <int CTRL>
+FDT_DEV double dppd(double v) {
+  int2 b = __builtin_bit_cast(int2, v);
+  b.x = __builtin_amdgcn_update_dpp(0, b.x, CTRL, 0xF, 0xF, false);
+  b.y = __builtin_amdgcn_update_dpp(0, b.y, CTRL, 0xF, 0xF, false);
+  return __builtin_bit_cast(double, b);
+}
+FDT_DEV double rld(double v, int lane) {
+  int2 b = __builtin_bit_cast(int2, v);
+  b.x = __builtin_amdgcn_readlane(b.x, lane);
+  b.y = __builtin_amdgcn_readlane(b.y, lane);
+  return __builtin_bit_cast(double, b);
+}
+FDT_DEV double wave_sum_d(double v) {
+  v += dppd<0xB1>(v);
+  v += dppd<0x4E>(v);
+  v += dppd<0x141>(v);
+  v += dppd<0x140>(v);
+  return (rld(v, 0) + rld(v, 16)) + (rld(v, 32) + rld(v, 48));
+}
+// make a per-lane-identical value provably wave-uniform
+FDT_DEV float uni(float v) { return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, v))); }
+FDT_DEV double uni(double v) {
+  int2 b = __builtin_bit_cast(int2, v);
+  b.x = __builtin_amdgcn_readfirstlane(b.x);
+  b.y = __builtin_amdgcn_readfirstlane(b.y);
+  return __builtin_bit_cast(double, b);
+}
+
+// ----------------------------------------------------------------- Philox
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11).  Draw layout:
+// key = (lo32 seed, hi32 seed), counter = (lo32 voxel, hi32 voxel, jump, slot).
+struct u4 { uint32_t x, y, z, w; };
+FDT_DEV u4 philox4x32_10(u4 c, uint32_t k0, uint32_t k1) {
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    uint32_t hi0 = __umulhi(0xD2511F53u, c.x), lo0 = 0xD2511F53u * c.x;
+    uint32_t hi1 = __umulhi(0xCD9E8D57u, c.z), lo1 = 0xCD9E8D57u * c.z;
+    u4 n;
+    n.x = hi1 ^ c.y ^ k0;
+    n.y = lo1;
+    n.z = hi0 ^ c.w ^ k1;
+    n.w = lo0;
+    c = n;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+  return c;
+}
+// Standard normal by Box-Muller (cosine branch) and log of an acceptance
+// uniform in (0,1], from one Philox block (24-bit uniforms).
+FDT_DEV void draw(uint64_t seed, uint64_t vox, uint32_t jump, uint32_t slot, float& n, float& logu) {
+  u4 c{(uint32_t)vox, (uint32_t)(vox >> 32), jump, slot};
+  u4 x = philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+  const float s24 = 1.0f / 16777216.0f;
+  float u1 = (float)((x.x >> 8) + 1u) * s24;
+  float u2 = (float)(x.y >> 8) * s24;
+  float u3 = (float)((x.z >> 8) + 1u) * s24;
+  n = sqrtf(-2.0f * logf(u1)) * cosf(6.283185307179586f * u2);
+  logu = logf(u3);
+}
+
+// ----------------------------------------------------------- transforms
+template <class T> FDT_DEV T f2x(T f) { return tan(f / T(TWO_PI_D)); }
+template <class T> FDT_DEV T x2f(T x) { return fabs(T(TWO_PI_D) * atan(x)); }
+template <class T> FDT_DEV T f2beta(T f) { return asin(sqrt(f)); }
+template <class T> FDT_DEV T beta2f(T b) { T s = sin(b); return s * s; }
+template <class T> FDT_DEV T sgn(T x) { return x > T(0) ? T(1) : (x < T(0) ? T(-1) : T(0)); }
+
+}  // namespace fdt

@@ -1,0 +1,170 @@
+/* fdt_xfibres.h -- C ABI of the MI355X xfibres engine (libfdt_xfibres.so).
+ *
+ * The engine replaces the reference's GPU library libbedpostx_cuda.so behind
+ * the xfibres_gpu host driver.  The reference library exports one C++ entry,
+ *     void xfibres_gpu(Matrix datam, Matrix bvecs, Matrix bvals, Matrix gradm,
+ *                      int idpart, int idSubpart, float seed, string subjdir)
+ * (CUDA/xfibres_gpu.h:15-23, CUDA/xfibres_gpu.cu:29-146), reads its options
+ * from the xfibresOptions singleton (CUDA/xfibres_gpu.cu:67) and returns its
+ * results only as files (CUDA/samples.cu:231-347).  Here the same pipeline is
+ * split into plain-pointer calls so a caller controls residency:
+ *
+ *   fdt_create / fdt_set_acquisition / fdt_load_data   -- the inputs
+ *   fdt_fit      -- LM initialisation  (CUDA/xfibres_gpu.cu:173-348 fit(),
+ *                   CUDA/diffmodels.cu:26-347, the three PVM LM kernels)
+ *   fdt_mcmc     -- burn-in + record   (CUDA/runmcmc.cu:27-316,
+ *                   CUDA/runmcmc_kernels.cu:193-1474)
+ *   fdt_finish   -- fibre sort, means, dyads (CUDA/samples.cu:134-347,
+ *                   make_dyadic_vectors.cc:60-88)
+ *   fdt_get_outputs -- device -> caller buffers (CUDA/xfibres_gpu.cu:659-738)
+ *
+ * Conventions: every call returns 0 or a negative FDT_E* code and never exits;
+ * fdt_last_error() gives the message.  One context per host thread; a context
+ * owns one HIP stream and all device buffers; caller buffers are host memory
+ * and are only read or written during the call that takes them.
+ *
+ * Sample layout (host and device): per-fibre samples q[k][s][v]
+ * (K x nsamples x nvox); per-voxel scalars q[s][v].  Each q[k] block is the
+ * reference's [nsamples][nvox] row-major "<name>samples" part file.
+ */
+#ifndef FDT_XFIBRES_H
+#define FDT_XFIBRES_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FDT_OK 0
+#define FDT_EINVAL (-1)   /* bad argument or shape */
+#define FDT_EHIP (-2)     /* HIP runtime error */
+#define FDT_ESTATE (-3)   /* call out of order (e.g. mcmc before fit) */
+#define FDT_ENOMEM (-4)   /* device allocation failed */
+#define FDT_EUNSUP (-5)   /* option combination not built */
+
+#define FDT_LAYOUT_VOX_MAJOR 0 /* data[v*N + i] */
+#define FDT_LAYOUT_VOL_MAJOR 1 /* data[i*nvox + v]: the reference data_<part> file layout */
+
+/* The xfibres option surface (xfibresoptions.h:25-56, defaults :104-148). */
+typedef struct fdt_opts {
+  int nfibres;             /* --nf (1..3) */
+  int model;               /* --model 1/2/3 */
+  float fudge;             /* --fudge */
+  int njumps;              /* --nj */
+  int nburn;               /* --bi */
+  int sampleevery;         /* --se */
+  int updateproposalevery; /* --upe */
+  uint64_t seed;           /* --seed */
+  int no_ard;              /* --noard */
+  int all_ard;             /* --allard */
+  int nonlin;              /* --nonlinear */
+  int cnonlin;             /* --cnonlinear */
+  int rician;              /* --rician */
+  int f0;                  /* --f0 */
+  int ardf0;               /* --ardf0 */
+  float R_prior_mean;      /* --Rmean */
+  float R_prior_std;       /* --Rstd */
+  float R_prior_fudge;     /* --Rfudge */
+  int lm_fp64;             /* 0: FP32 model/Jacobian (production); 1: FP64 throughout (parity check) */
+  int reserved[7];
+} fdt_opts;
+
+/* Output buffers; any pointer may be NULL to skip that output. */
+typedef struct fdt_outputs {
+  float* th;        /* K x ns x nvox, fibres sorted by descending sum_s f */
+  float* ph;        /* K x ns x nvox */
+  float* f;         /* K x ns x nvox */
+  float* mean_S0;   /* nvox */
+  float* mean_d;    /* nvox */
+  float* mean_dstd; /* nvox (model >= 2) */
+  float* mean_R;    /* nvox (model 3) */
+  float* mean_f0;   /* nvox (--f0) */
+  float* mean_tau;  /* nvox (--rician) */
+  float* mean_f;    /* K x nvox */
+  float* dyads;     /* K x 3 x nvox: principal eigenvector of the mean dyadic tensor */
+  float* disp;      /* K x nvox: 1 - largest eigenvalue of the mean dyadic tensor */
+  float* S0;        /* ns x nvox raw samples (optional) */
+  float* d;         /* ns x nvox */
+  float* dstd;      /* ns x nvox */
+  float* R;         /* ns x nvox */
+  float* f0;        /* ns x nvox */
+  float* tau;       /* ns x nvox */
+} fdt_outputs;
+
+typedef struct fdt_timing {
+  float ms_fit;     /* device time of the LM kernels of the last fdt_fit */
+  float ms_mcmc;    /* device time of the MCMC kernel of the last fdt_mcmc */
+  float ms_finish;  /* device time of the finish kernel of the last fdt_finish */
+  int mcmc_vpl;     /* volumes per lane chosen for the MCMC kernel */
+  int mcmc_lanes;   /* lanes per voxel */
+} fdt_timing;
+
+typedef struct fdt_ctx fdt_ctx;
+
+/* Fill o with the xfibres defaults (xfibresoptions.h:104-148). */
+void fdt_default_opts(fdt_opts* o);
+
+/* Create a context on HIP device `device` (replaces init_gpu,
+ * CUDA/init_gpu.cu:12-25, and the options singleton). NULL on failure. */
+fdt_ctx* fdt_create(int device, const fdt_opts* o);
+void fdt_destroy(fdt_ctx* c);
+const char* fdt_last_error(const fdt_ctx* c);
+
+/* bvals [N], bvecs 3 x N row-major (x row, y row, z row), already unit
+ * length as xfibres_gpu.cc:41-53 leaves them (replaces the bvals/bvecs
+ * arguments of xfibres_gpu(), CUDA/xfibres_gpu.h:17-18). */
+int fdt_set_acquisition(fdt_ctx* c, int N, const float* bvals, const float* bvecs);
+
+/* Copy nvox voxels of N volumes to the device (replaces datam,
+ * CUDA/xfibres_gpu.h:16, and prepare_data_gpu_FIT, CUDA/xfibres_gpu.cu:370-461).
+ * `layout` is FDT_LAYOUT_VOL_MAJOR for the data_<part> file order. */
+int fdt_load_data(fdt_ctx* c, const float* data, int layout, int nvox);
+
+/* Number of LM output parameters per voxel: 2 + 3K (+1 model>=2) (+1 --f0);
+ * layout [S0, d, (d_std), f_k, th_k, ph_k ..., (f0)] (CUDA/xfibres_gpu.cu:450-454). */
+int fdt_num_params(const fdt_ctx* c);
+
+/* LM initialisation on the resident data (CUDA/xfibres_gpu.cu:173-348 and
+ * calculate_tau, CUDA/diffmodels.cu:259-347).  Tensor start values are
+ * computed on the device.  Results stay resident. */
+int fdt_fit(fdt_ctx* c);
+
+/* One LM stage from caller-given start values, for parity checks of the LM
+ * kernels alone (fit_PVM_single_kernel CUDA/PVM_single.cu:387-473,
+ * fit_PVM_single_c_kernel CUDA/PVM_single_c.cu:516-616,
+ * fit_PVM_multi_kernel CUDA/PVM_multi.cu:468-576).
+ * kind 0 PVM_single, 1 PVM_single_c, 2 PVM_multi (G = Gamma_for_ball_only,
+ * R = model-3 ratio); start/out [nvox][np] with np = (kind==2?3:2)+3K+f0;
+ * start in the LM's transformed space, out finalised (as the kernels leave
+ * it). iters may be NULL. Uses the resident data. */
+int fdt_lm_stage(fdt_ctx* c, int kind, int K, int f0, int G, float R, const float* start, float* out, int* iters);
+
+/* Device tensor start values of one LM stage (CUDA/diffmodels.cu:57-93,
+ * 149-183): out [nvox][np] for kind 0 (PVM_single) or 1 (PVM_single_c). */
+int fdt_lm_start(fdt_ctx* c, int kind, int K, int f0, float* out);
+
+int fdt_get_params(fdt_ctx* c, float* params, float* tau);
+int fdt_set_params(fdt_ctx* c, const float* params, const float* tau);
+
+/* Burn-in + record as one persistent launch (CUDA/runmcmc.cu:108-316).
+ * Draws are Philox4x32-10 keyed by --seed and countered by
+ * (vox_offset + voxel, jump, proposal), so results do not depend on how the
+ * voxels are split into calls. */
+int fdt_mcmc(fdt_ctx* c, uint64_t vox_offset);
+
+int fdt_finish(fdt_ctx* c);
+int fdt_get_outputs(fdt_ctx* c, const fdt_outputs* out);
+
+/* fit + mcmc + finish on the resident data, asynchronous on the context stream. */
+int fdt_run(fdt_ctx* c, uint64_t vox_offset);
+int fdt_sync(fdt_ctx* c);
+int fdt_get_timing(fdt_ctx* c, fdt_timing* t);
+
+/* Force the MCMC volumes-per-lane tile (0 = automatic). */
+int fdt_set_mcmc_tile(fdt_ctx* c, int vpl);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
