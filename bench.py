@@ -1,0 +1,145 @@
+"""Throughput of the MI355X xfibres engine on the multi-shell 3-fibre workload.
+
+One step = one pass of the hot path over the whole synthetic volume with the
+inputs resident in HBM: LM initialisation (tensor start values + PVM_single +
+PVM_multi), 1000 burn-in + 1250 recorded MCMC sweeps, and the finishing pass
+(fibre sort, means, dyads).  Workload: model 2 (sticks with gamma-distributed
+diffusivities), nf=3, 18 b=0 + 90 directions at each of b=1000/2000/3000
+(N=288), an ellipsoid mask of 250,968 voxels in a 110x110x72 grid, Rician
+SNR 25, synthetic (seeded) data.
+
+Also reported: the roofline position of the MCMC kernel (algorithmic FP32
+flops per launch / its HIP-event duration, against 157.3 TFLOP/s) and the
+CPU baseline: the oracle's FP32 restatement of the reference CPU xfibres,
+timed on this host on a bounded sample of the same voxels.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+FP32_PEAK_TFLOPS = 157.3
+METRIC = "voxels/s, 3-fibre multi-shell bedpostx; surrogate vs plain at equal dyad error"
+
+
+def flops_per_jump(model: int, K: int, N: int, f0: bool = False, rician: bool = False) -> int:
+    """Algorithmic FP32 flops of one MCMC sweep of one voxel.
+
+    F = N * (P*L + R_sig): P proposals each re-evaluating the likelihood
+    (L = 2K+7 per volume, Gaussian; 2K+33 Rician) plus the signal updates
+    (d: K*A + I; d_std: K*A + I for model 2, I for model 3; R: K*A; 2K angle
+    proposals: 6 + A each), with A = 3/7/5 and I = 2/6/6 for models 1/2/3.
+    FMA = 2, add/mul = 1, each exp/log/div/sqrt = 1."""
+    P = 3 * K + 2 + (model >= 2) + (model == 3) + int(f0) + int(rician)
+    L = 2 * K + (33 if rician else 7)
+    A = {1: 3, 2: 7, 3: 5}[model]
+    I = {1: 2, 2: 6, 3: 6}[model]
+    R = K * A + I
+    if model >= 2:
+        R += (K * A + I) if model == 2 else I
+    if model == 3:
+        R += K * A
+    R += 2 * K * (6 + A)
+    return N * (P * L + R)
+
+
+def cpu_baseline(v, opts, seconds=15.0, threads=None):
+    from oracle import oracle as O
+    threads = threads or min(16, os.cpu_count() or 1)
+    O.set_threads(threads)
+    o = O.opts(nfibres=opts["nfibres"], model=opts["model"], nburn=opts["nburn"], njumps=opts["njumps"],
+               sampleevery=opts["sampleevery"])
+    rng = np.random.default_rng(12345)
+    n = max(threads, 16)
+    done, t_used = 0, 0.0
+    while t_used < seconds and done < 4096:
+        idx = rng.choice(v["data"].shape[0], n, replace=False)
+        d = v["data"][idx].astype(np.float64)
+        t = time.perf_counter()
+        p, tau, _ = O.fit(o, v["bvals"], v["bvecs"], d, prec=32)
+        O.mcmc(o, v["bvals"], v["bvecs"], d, p, tau, prec=32)
+        t_used += time.perf_counter() - t
+        done += n
+        n = min(2 * n, 4096 - done) if t_used < seconds / 3 else n
+    return dict(value=done / t_used, unit="voxels/s", cores=threads, kind="port",
+                sample=f"{done} voxels drawn at random from the workload, full LM + {opts['nburn']}+{opts['njumps']} "
+                       f"MCMC sweeps each, oracle FP32 restatement of xfibres on {threads} OpenMP threads "
+                       f"({t_used:.1f} s of CPU work)")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--nvox", type=int, default=0, help="override the voxel count (default: the full mask)")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: --gpus {args.gpus} is not supported: this engine runs on one GPU per process "
+              "(voxels shard by part across processes as bedpostx_gpu does); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import fdt_amd as F
+    from fdt_amd import synth
+
+    nvox = args.nvox or synth.C1_NVOX
+    v = synth.voxels("c1", nvox, with_truth=False)
+    N = v["bvals"].size
+    opts = dict(nfibres=3, model=2, nburn=1000, njumps=1250, sampleevery=25)
+    x = F.Xfibres(**opts)
+    x.set_acquisition(v["bvals"], v["bvecs"])
+    x.load_data(v["data"])
+
+    for _ in range(args.warmup):
+        x.run(0)
+        x.sync()
+    ms_mcmc, ms_fit, ms_fin = [], [], []
+    x.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        x.run(0)
+        x.sync()
+        t = x.timing()
+        ms_mcmc.append(t["ms_mcmc"]); ms_fit.append(t["ms_fit"]); ms_fin.append(t["ms_finish"])
+    x.sync()
+    elapsed = time.perf_counter() - t0
+    ms_step = elapsed / args.steps * 1e3
+    value = nvox / (ms_step / 1e3)
+    tm = x.timing()
+    x.close()
+
+    fl_launch = flops_per_jump(2, 3, N) * (opts["nburn"] + opts["njumps"]) * nvox
+    mc_ms = float(np.mean(ms_mcmc))
+    achieved = fl_launch / (mc_ms / 1e3) / 1e12
+    line = {
+        "metric": METRIC, "value": value, "unit": "voxels/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "fp32",
+        "data": "synthetic (seeded Philox), no published dataset",
+        "config": {"workload": f"multi-shell model 2, nf=3, N={N} (18 b0 + 3x90 at b=1000/2000/3000), "
+                               f"{nvox} voxels, bi=1000 nj=1250 se=25, plain LM init",
+                   "init": "plain LM (tensor start -> PVM_single -> PVM_multi)",
+                   "ms_fit": float(np.mean(ms_fit)), "ms_mcmc": mc_ms, "ms_finish": float(np.mean(ms_fin)),
+                   "mcmc_tile": f"{tm['lanes']} lanes x {tm['vpl']} volumes per voxel"},
+        "roofline": {"bound": "valu-fp32", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None,
+                     "kernel": "mcmc_kernel<3,18,false>",
+                     "flops_per_launch": fl_launch},
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(v, opts, seconds=args.cpu_seconds)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

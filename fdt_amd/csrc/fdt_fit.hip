@@ -21,10 +21,6 @@
 namespace fdt {
 
 FDT_DEV int lane_id() { return __lane_id(); }
-FDT_DEV void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-}
 
 // The voxel's volumes, streamed: the acquisition (gx, gy, gz, b) is staged in
 // LDS for the workgroup, the signal row is read from the voxel-major data.
@@ -301,63 +297,80 @@ FDT_DEV double lm_cost(const Vols& V, const T* p, int G, T Rv, T invR) {
   return uni(wave_sum_d(c));
 }
 
-// LU with partial pivoting + substitution on the wave's LDS workspace,
-// executed by lane 0 (CUDA/solver_mult_inverse.cu:14-166).
-FDT_DEV void lu_solve_lds(int n, const double* A, const double* b, double* C, double* x) {
-  int indx[NPMAX];
-  for (int i = 0; i < n * n; ++i) C[i] = A[i];
+// LU with partial pivoting and substitution, distributed over the wave: lane
+// i (< NP) holds row i of the matrix and entry i of the right-hand side.  The
+// elimination order, the pivot rule (first maximum of the updated column
+// below the diagonal; a zero pivot skips elimination) and the interchange of
+// the right-hand side follow CUDA/solver_mult_inverse.cu:14-166.  Pivot
+// search, row swaps and the pivot-row broadcast use v_readlane, so the solve
+// stays in registers.
+template <int NP>
+FDT_DEV void lu_rows(const double (&A)[NP], double b, double (&x)[NP]) {
+  const int lane = lane_id();
+  double r[NP];
+#pragma unroll
+  for (int l = 0; l < NP; ++l) r[l] = A[l];
+  double rb = b;
   int mu = 0;
-  double big = fabs(C[0]);
-  for (int k = 1; k < n; ++k)
-    if (big < fabs(C[k * n])) { big = fabs(C[k * n]); mu = k; }
-  for (int k = 0; k < n; ++k) {
-    indx[k] = mu;
-    if (mu != k)
-      for (int j = 0; j < n; ++j) { double t = C[k * n + j]; C[k * n + j] = C[mu * n + j]; C[mu * n + j] = t; }
-    double diag = C[k * n + k];
-    big = 0.0;
+  double big = fabs(rld(r[0], 0));
+#pragma unroll
+  for (int i = 1; i < NP; ++i) {
+    double t = fabs(rld(r[0], i));
+    if (big < t) { big = t; mu = i; }
+  }
+#pragma unroll
+  for (int k = 0; k < NP; ++k) {
+    if (mu != k) {
+#pragma unroll
+      for (int l = 0; l < NP; ++l) {
+        double vk = rld(r[l], k), vm = rld(r[l], mu);
+        r[l] = lane == k ? vm : (lane == mu ? vk : r[l]);
+      }
+      double bk = rld(rb, k), bm = rld(rb, mu);
+      rb = lane == k ? bm : (lane == mu ? bk : rb);
+    }
+    double pr[NP];
+#pragma unroll
+    for (int l = k; l < NP; ++l) pr[l] = rld(r[l], k);
+    const double diag = pr[k];
     mu = k + 1;
+    big = 0.0;
     if (diag != 0.0) {
-      for (int i = k + 1; i < n; ++i) {
-        double mult = C[i * n + k] / diag;
-        C[i * n + k] = mult;
-        for (int l = k + 1; l < n; ++l) {
-          double v = C[i * n + l] - mult * C[k * n + l];
-          C[i * n + l] = v;
-          if (l == k + 1 && big < fabs(v)) { big = fabs(v); mu = i; }
-        }
+      if (lane > k && lane < NP) {
+        double mult = r[k] / diag;
+        r[k] = mult;
+#pragma unroll
+        for (int l = k + 1; l < NP; ++l) r[l] = r[l] - mult * pr[l];
+      }
+      const int kn = k + 1 < NP ? k + 1 : k;
+#pragma unroll
+      for (int i = k + 1; i < NP; ++i) {
+        double t = fabs(rld(r[kn], i));
+        if (big < t) { big = t; mu = i; }
       }
     }
   }
-  for (int i = 0; i < n; ++i) x[i] = b[i];
-  int ii = n;
-  for (int i = 0; i < n; ++i) {
-    int ip = indx[i];
-    double t = x[ip]; x[ip] = x[i]; x[i] = t;
-    if (t != 0.0) { ii = i; break; }
+  // forward substitution with the unit lower factor (lane i: L[i][j] = r[j], j < i)
+  double acc = rb;
+#pragma unroll
+  for (int j = 0; j < NP; ++j) {
+    double yj = rld(acc, j);
+    if (lane > j && lane < NP) acc = acc - r[j] * yj;
   }
-  for (int i = ii + 1; i < n; ++i) {
-    int ip = indx[i];
-    double sum = x[ip]; x[ip] = x[i];
-    for (int j = ii; j < i; ++j) sum -= C[i * n + j] * x[j];
-    x[i] = sum;
-  }
-  for (int i = n - 1; i >= 0; --i) {
-    double sum = x[i];
-    for (int j = i + 1; j < n; ++j) sum -= C[i * n + j] * x[j];
-    x[i] = sum / C[i * n + i];
+  // back substitution with the upper factor
+#pragma unroll
+  for (int i = NP - 1; i >= 0; --i) {
+    double xi = rld(acc, i) / rld(r[i], i);
+    x[i] = xi;
+    if (lane < i) acc = acc - r[i] * xi;
   }
 }
-
-struct WaveWS {  // per-wave LDS workspace
-  double H[NPMAX * NPMAX], C[NPMAX * NPMAX], g[NPMAX], sol[NPMAX];
-};
 
 // The Levenberg loop of CUDA/levenberg_marquardt.cu:24-115: lambda0 = 0.1,
 // /10 after a success, x10 after a failure (the Hessian diagonal carries
 // lambda - lambda_old), relative cost test 1e-8, ltol 1e20, 200 successes.
 template <class T, int KIND, int K, bool F0>
-FDT_DEV int lm_stage(const Vols& V, T* p, int G, T Rv, T invR, WaveWS* ws) {
+FDT_DEV int lm_stage(const Vols& V, T* p, int G, T Rv, T invR) {
   using M = Mdl<T, KIND, K, F0>;
   constexpr int NP = M::NP;
   constexpr int NH = NP * (NP + 1) / 2;
@@ -368,6 +381,9 @@ FDT_DEV int lm_stage(const Vols& V, T* p, int G, T Rv, T invR, WaveWS* ws) {
   int niter = 0, total = 0;
   double pcf = lm_cost<T, KIND, K, F0>(V, p, G, Rv, invR), ncf;
   T step[NP];
+  double Hrow[NP], grow = 0.0;  // lane i < NP: row i of the damped Hessian, gradient entry i
+#pragma unroll
+  for (int i = 0; i < NP; ++i) Hrow[i] = 0.0;
   while (!(success && niter++ >= 200)) {
     ++total;
     if (success) {
@@ -402,7 +418,7 @@ FDT_DEV int lm_stage(const Vols& V, T* p, int G, T Rv, T invR, WaveWS* ws) {
 #pragma unroll
       for (int a = 0; a < NP; ++a) {
         double ga = (double)wsum<T>(gacc[a]);
-        if (lane == 0) ws->g[a] = ga;
+        grow = lane == a ? ga : grow;
       }
       int h = 0;
 #pragma unroll
@@ -410,17 +426,16 @@ FDT_DEV int lm_stage(const Vols& V, T* p, int G, T Rv, T invR, WaveWS* ws) {
 #pragma unroll
         for (int c = a; c < NP; ++c) {
           double hv = (double)wsum<T>(hacc[h++]);
-          if (lane == 0) { ws->H[a * NP + c] = hv; ws->H[c * NP + a] = hv; }
+          if (lane == a) Hrow[c] = hv;
+          if (lane == c) Hrow[a] = hv;
         }
     }
-    if (lane == 0) {
-      for (int i = 0; i < NP; ++i) ws->H[i * NP + i] += (lambda - olambda);
-      lu_solve_lds(NP, ws->H, ws->g, ws->C, ws->sol);
-    }
-    wave_sync();
 #pragma unroll
-    for (int i = 0; i < NP; ++i) step[i] = p[i] - (T)uni(ws->sol[i]);
-    wave_sync();
+    for (int i = 0; i < NP; ++i) if (lane == i) Hrow[i] += (lambda - olambda);
+    double sol[NP];
+    lu_rows<NP>(Hrow, grow, sol);
+#pragma unroll
+    for (int i = 0; i < NP; ++i) step[i] = p[i] - (T)sol[i];
     ncf = lm_cost<T, KIND, K, F0>(V, step, G, Rv, invR);
     if ((success = (ncf < pcf))) {
       olambda = 0.0;
@@ -791,17 +806,17 @@ FDT_DEV void start_multi(const T* s1, T* p) {
 // then PVM_multi (refit after an implausible --f0 fit: PVM_single_c then
 // PVM_multi, CUDA/xfibres_gpu.cu:331-333).  Output [S0,d,(dstd),f,th,ph..,(f0)].
 template <class T, int K, bool F0>
-FDT_DEV int run_model1(const Vols& V, const Dti& dti, bool nonlin, T* out, WaveWS* ws) {
+FDT_DEV int run_model1(const Vols& V, const Dti& dti, bool nonlin, T* out) {
   constexpr int NP = 2 + 3 * K + (F0 ? 1 : 0);
   T p[NP];
   int it;
   if (nonlin) {
     start_single<T, K, F0>(dti, p);
-    it = lm_stage<T, KIND_SINGLE, K, F0>(V, p, 0, T(0), T(1), ws);
+    it = lm_stage<T, KIND_SINGLE, K, F0>(V, p, 0, T(0), T(1));
     finalise<T, KIND_SINGLE, K, F0>(p);
   } else {
     start_single_c<T, K, F0>(dti, V, p);
-    it = lm_stage<T, KIND_SINGLE_C, K, F0>(V, p, 0, T(0), T(1), ws);
+    it = lm_stage<T, KIND_SINGLE_C, K, F0>(V, p, 0, T(0), T(1));
     finalise<T, KIND_SINGLE_C, K, F0>(p);
   }
 #pragma unroll
@@ -810,22 +825,22 @@ FDT_DEV int run_model1(const Vols& V, const Dti& dti, bool nonlin, T* out, WaveW
 }
 
 template <class T, int K, bool F0>
-FDT_DEV int run_multi(const Vols& V, const Dti& dti, bool refit, int G, T Rv, T* out, WaveWS* ws) {
+FDT_DEV int run_multi(const Vols& V, const Dti& dti, bool refit, int G, T Rv, T* out) {
   constexpr int NP1 = 2 + 3 * K + (F0 ? 1 : 0), NP = NP1 + 1;
   T s1[NP1], p[NP];
   int it;
   if (!refit) {
     start_single<T, K, F0>(dti, s1);
-    it = lm_stage<T, KIND_SINGLE, K, F0>(V, s1, 0, T(0), T(1), ws);
+    it = lm_stage<T, KIND_SINGLE, K, F0>(V, s1, 0, T(0), T(1));
     finalise<T, KIND_SINGLE, K, F0>(s1);
   } else {
     start_single_c<T, K, F0>(dti, V, s1);
-    it = lm_stage<T, KIND_SINGLE_C, K, F0>(V, s1, 0, T(0), T(1), ws);
+    it = lm_stage<T, KIND_SINGLE_C, K, F0>(V, s1, 0, T(0), T(1));
     finalise<T, KIND_SINGLE_C, K, F0>(s1);
   }
   start_multi<T, K, F0>(s1, p);
   T invR = T(1) / (T(2) * Rv + T(1));
-  it += lm_stage<T, KIND_MULTI, K, F0>(V, p, G, Rv, invR, ws);
+  it += lm_stage<T, KIND_MULTI, K, F0>(V, p, G, Rv, invR);
   finalise<T, KIND_MULTI, K, F0>(p);
 #pragma unroll
   for (int i = 0; i < NP; ++i) out[i] = p[i];
@@ -879,7 +894,7 @@ FDT_DEV T tau_of(const Vols& V, int model, bool f0, const T* q, T Rv) {
 }
 
 template <class T, int K>
-FDT_DEV void fit_pipeline(const FitArgs& a, const Vols& V, int vox, WaveWS* ws) {
+FDT_DEV void fit_pipeline(const FitArgs& a, const Vols& V, int vox) {
   const int lane = lane_id();
   Dti dti = dti_fit(V, a.pinvA);
   T out[NPMAX];
@@ -889,28 +904,28 @@ FDT_DEV void fit_pipeline(const FitArgs& a, const Vols& V, int vox, WaveWS* ws) 
   T Rv = (T)a.Rmean;
   if (a.model == 1) {
     if (a.f0) {
-      it = run_model1<T, K, true>(V, dti, a.nonlin, out, ws);
+      it = run_model1<T, K, true>(V, dti, a.nonlin, out);
       T md = out[1], mf = out[2], f0v = out[npo - 1];
       if ((K > 0 && mf < T(0.05)) || md > T(0.007) || f0v > T(0.4)) {
-        it += run_model1<T, K, false>(V, dti, a.nonlin, out, ws);
+        it += run_model1<T, K, false>(V, dti, a.nonlin, out);
         out[npo - 1] = T(0.001);
         used_f0 = false;
       }
     } else {
-      it = run_model1<T, K, false>(V, dti, a.nonlin, out, ws);
+      it = run_model1<T, K, false>(V, dti, a.nonlin, out);
     }
   } else {
     int G = a.model == 3 ? 2 : 0;
     if (a.f0) {
-      it = run_multi<T, K, true>(V, dti, false, G, Rv, out, ws);
+      it = run_multi<T, K, true>(V, dti, false, G, Rv, out);
       T md = out[1], mf = out[3], f0v = out[npo - 1];
       if ((K > 0 && mf < T(0.05)) || md > T(0.007) || f0v > T(0.4)) {
-        it += run_multi<T, K, false>(V, dti, true, G, Rv, out, ws);
+        it += run_multi<T, K, false>(V, dti, true, G, Rv, out);
         out[npo - 1] = T(0.001);
         used_f0 = false;
       }
     } else {
-      it = run_multi<T, K, false>(V, dti, false, G, Rv, out, ws);
+      it = run_multi<T, K, false>(V, dti, false, G, Rv, out);
     }
   }
   T tau = T(0);
@@ -928,14 +943,14 @@ FDT_DEV void fit_pipeline(const FitArgs& a, const Vols& V, int vox, WaveWS* ws) 
 }
 
 template <class T, int K, int KIND, bool F0>
-FDT_DEV void fit_stage(const FitArgs& a, const Vols& V, int vox, WaveWS* ws) {
+FDT_DEV void fit_stage(const FitArgs& a, const Vols& V, int vox) {
   constexpr int NP = Mdl<T, KIND, K, F0>::NP;
   const int lane = lane_id();
   T p[NP];
 #pragma unroll
   for (int i = 0; i < NP; ++i) p[i] = (T)a.start[(size_t)vox * NP + i];
   T Rv = (T)a.R;
-  int it = lm_stage<T, KIND, K, F0>(V, p, a.G, Rv, T(1) / (T(2) * Rv + T(1)), ws);
+  int it = lm_stage<T, KIND, K, F0>(V, p, a.G, Rv, T(1) / (T(2) * Rv + T(1)));
   finalise<T, KIND, K, F0>(p);
   if (lane < NP) {
     T v = p[0];
@@ -965,7 +980,6 @@ FDT_DEV void fit_start(const FitArgs& a, const Vols& V, int vox) {
 template <int K, bool FP64>
 __global__ __launch_bounds__(64 * FIT_WAVES) void fit_kernel(FitArgs a) {
   using T = typename std::conditional<FP64, double, float>::type;
-  __shared__ WaveWS wss[FIT_WAVES];
   __shared__ float4 sacq[MAX_N];
   for (int i = threadIdx.x; i < a.N; i += blockDim.x)
     sacq[i] = make_float4(a.bvecs[i], a.bvecs[a.N + i], a.bvecs[2 * a.N + i], a.bvals[i]);
@@ -974,7 +988,6 @@ __global__ __launch_bounds__(64 * FIT_WAVES) void fit_kernel(FitArgs a) {
   const int lane = lane_id();
   const int vox = blockIdx.x * FIT_WAVES + wave;
   if (vox >= a.nvox) return;
-  WaveWS* ws = &wss[wave];
   Vols V;
   V.y = a.data + (size_t)vox * a.N;
   V.acq = sacq;
@@ -993,17 +1006,17 @@ __global__ __launch_bounds__(64 * FIT_WAVES) void fit_kernel(FitArgs a) {
     V.ysub = mnp;
   }
   if (a.mode == FIT_PIPELINE) {
-    fit_pipeline<T, K>(a, V, vox, ws);
+    fit_pipeline<T, K>(a, V, vox);
   } else if (a.mode == FIT_STAGE) {
     if (a.kind == KIND_SINGLE) {
-      if (a.stage_f0) fit_stage<T, K, KIND_SINGLE, true>(a, V, vox, ws);
-      else fit_stage<T, K, KIND_SINGLE, false>(a, V, vox, ws);
+      if (a.stage_f0) fit_stage<T, K, KIND_SINGLE, true>(a, V, vox);
+      else fit_stage<T, K, KIND_SINGLE, false>(a, V, vox);
     } else if (a.kind == KIND_SINGLE_C) {
-      if (a.stage_f0) fit_stage<T, K, KIND_SINGLE_C, true>(a, V, vox, ws);
-      else fit_stage<T, K, KIND_SINGLE_C, false>(a, V, vox, ws);
+      if (a.stage_f0) fit_stage<T, K, KIND_SINGLE_C, true>(a, V, vox);
+      else fit_stage<T, K, KIND_SINGLE_C, false>(a, V, vox);
     } else {
-      if (a.stage_f0) fit_stage<T, K, KIND_MULTI, true>(a, V, vox, ws);
-      else fit_stage<T, K, KIND_MULTI, false>(a, V, vox, ws);
+      if (a.stage_f0) fit_stage<T, K, KIND_MULTI, true>(a, V, vox);
+      else fit_stage<T, K, KIND_MULTI, false>(a, V, vox);
     }
   } else {
     if (a.stage_f0) fit_start<T, K, true>(a, V, vox);

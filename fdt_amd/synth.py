@@ -12,7 +12,7 @@ Workloads (voxel counts follow the masks they are quoted on):
         p=0.5, f2~U(0.1,f1), crossing U(30,90) deg, Rician SNR 20.
   "c1"  multi-shell: 18 b=0 + 90 dirs at each of b=1000/2000/3000 (N=288),
         110x110x72 grid with an ellipsoid mask of semi-axes (48,48,26)
-        (250,930 voxels); d~U(0.8e-3,2e-3), d_std=d*U(0.1,0.5), 1-3 fibres
+        (250,968 voxels); d~U(0.8e-3,2e-3), d_std=d*U(0.1,0.5), 1-3 fibres
         with weights summing to U(0.3,0.8), crossings U(30,90) deg, SNR 25.
   "c2_33", "c2_69", "c2_145", "c2_288": the mixed-acquisition volumes.
 """
@@ -81,12 +81,29 @@ def _sph(v: np.ndarray):
     return th, ph
 
 
+CHUNK = 16384
+
+
 def voxels(name: str, nvox: int, sub: int = 0, with_truth: bool = True):
-    """Returns dict(bvals, bvecs, data [nvox, N] float32 voxel-major, truth...)."""
+    """Returns dict(bvals, bvecs, data [nvox, N] float32 voxel-major, truth...).
+
+    Voxels are drawn in fixed blocks of CHUNK, block c from Philox stream
+    (name, sub * 4096 + c): the result is a pure function of (name, nvox, sub),
+    and the first c*CHUNK voxels do not depend on nvox."""
     acq_g = rng(name, 1000 + sub)
     n_b0, shells = ACQ[name]
     bvals, bvecs = acquisition(n_b0, shells, acq_g)
-    g = rng(name, sub)
+    parts = []
+    for c0 in range(0, nvox, CHUNK):
+        parts.append(_voxel_block(name, min(CHUNK, nvox - c0), rng(name, 2000 + sub * 4096 + c0 // CHUNK), bvals, bvecs,
+                                  with_truth))
+    out = {k: (np.concatenate([p[k] for p in parts]) if isinstance(parts[0][k], np.ndarray) else parts[0][k])
+           for k in parts[0]}
+    out.update(bvals=bvals, bvecs=bvecs)
+    return out
+
+
+def _voxel_block(name, nvox, g, bvals, bvecs, with_truth):
     N = bvals.size
     K = 3
     S0 = g.uniform(800.0, 1200.0, nvox)
@@ -130,7 +147,7 @@ def voxels(name: str, nvox: int, sub: int = 0, with_truth: bool = True):
     n1 = g.standard_normal((nvox, N)) * sigma[:, None]
     n2 = g.standard_normal((nvox, N)) * sigma[:, None]
     data = np.sqrt((sig + n1) ** 2 + n2 ** 2).astype(np.float32)
-    out = dict(bvals=bvals, bvecs=bvecs, data=data, model=model)
+    out = dict(data=data, model=model)
     if with_truth:
         th, ph = _sph(dirs.reshape(-1, 3))
         out.update(S0=S0, d=d, dstd=dstd, f=f, dirs=dirs, th=th.reshape(nvox, K), ph=ph.reshape(nvox, K),

@@ -137,40 +137,59 @@ static void jacobi_eig(int n, const double* Ain, double* w, double* V) {
   std::memcpy(V, Vs, sizeof(double) * n * n);
 }
 
-// Moore-Penrose pseudo-inverse of an m x n matrix (n <= 8) through the
-// eigen-decomposition of A^T A; singular values below
+// Moore-Penrose pseudo-inverse of an m x n matrix (n <= 8) by one-sided
+// Jacobi SVD (Hestenes): columns are orthogonalised by plane rotations, the
+// column norms are the singular values.  Singular values at or below
 // max(m,n) * s_max * 1e-15 are treated as zero (the FSL pinv tolerance is
 // un-vendored; this matters only for rank-deficient input).
 static void pinv(int m, int n, const double* A, double* P /* n x m */) {
-  double AtA[64] = {0};
-  for (int i = 0; i < n; ++i)
-    for (int j = 0; j < n; ++j) {
-      double s = 0;
-      for (int k = 0; k < m; ++k) s += A[k * n + i] * A[k * n + j];
-      AtA[i * n + j] = s;
-    }
-  double w[8], V[64];
-  jacobi_eig(n, AtA, w, V);
-  double smax = std::sqrt(std::max(w[n - 1], 0.0));
-  double tol = std::max(m, n) * smax * 1e-15;
-  double inv[8];
-  for (int j = 0; j < n; ++j) {
-    double s = std::sqrt(std::max(w[j], 0.0));
-    inv[j] = (s > tol) ? 1.0 / (s * s) : 0.0;
+  std::vector<double> U((size_t)m * n);
+  std::memcpy(U.data(), A, sizeof(double) * m * n);
+  double V[64];
+  for (int i = 0; i < n * n; ++i) V[i] = 0;
+  for (int i = 0; i < n; ++i) V[i * n + i] = 1;
+  for (int sweep = 0; sweep < 60; ++sweep) {
+    bool rotated = false;
+    for (int p = 0; p < n - 1; ++p)
+      for (int q = p + 1; q < n; ++q) {
+        double al = 0, be = 0, ga = 0;
+        for (int k = 0; k < m; ++k) {
+          double up = U[(size_t)k * n + p], uq = U[(size_t)k * n + q];
+          al += up * up; be += uq * uq; ga += up * uq;
+        }
+        if (ga == 0 || std::fabs(ga) <= 1e-15 * std::sqrt(al * be)) continue;
+        rotated = true;
+        double zeta = (be - al) / (2 * ga);
+        double t = (zeta >= 0 ? 1.0 : -1.0) / (std::fabs(zeta) + std::sqrt(1 + zeta * zeta));
+        double c = 1 / std::sqrt(1 + t * t), sn = c * t;
+        for (int k = 0; k < m; ++k) {
+          double up = U[(size_t)k * n + p], uq = U[(size_t)k * n + q];
+          U[(size_t)k * n + p] = c * up - sn * uq;
+          U[(size_t)k * n + q] = sn * up + c * uq;
+        }
+        for (int k = 0; k < n; ++k) {
+          double vp = V[k * n + p], vq = V[k * n + q];
+          V[k * n + p] = c * vp - sn * vq;
+          V[k * n + q] = sn * vp + c * vq;
+        }
+      }
+    if (!rotated) break;
   }
-  // P = V diag(inv) V^T A^T
-  double M[64];
-  for (int i = 0; i < n; ++i)
-    for (int j = 0; j < n; ++j) {
-      double s = 0;
-      for (int k = 0; k < n; ++k) s += V[i * n + k] * inv[k] * V[j * n + k];
-      M[i * n + j] = s;
-    }
+  double sv[8], smax = 0;
+  for (int j = 0; j < n; ++j) {
+    double s = 0;
+    for (int k = 0; k < m; ++k) s += U[(size_t)k * n + j] * U[(size_t)k * n + j];
+    sv[j] = std::sqrt(s);
+    smax = std::max(smax, sv[j]);
+  }
+  double tol = std::max(m, n) * smax * 1e-15;
+  // P = V diag(1/s) U_normalised^T = V diag(1/s^2) U^T (U columns carry s)
   for (int i = 0; i < n; ++i)
     for (int k = 0; k < m; ++k) {
-      double s = 0;
-      for (int j = 0; j < n; ++j) s += M[i * n + j] * A[k * n + j];
-      P[i * m + k] = s;
+      double acc = 0;
+      for (int j = 0; j < n; ++j)
+        if (sv[j] > tol) acc += V[i * n + j] * U[(size_t)k * n + j] / (sv[j] * sv[j]);
+      P[i * m + k] = acc;
     }
 }
 
@@ -1286,7 +1305,7 @@ typedef struct orc_samples {
     return np;                                                                                                \
   }                                                                                                           \
   int PFX##_lu_solve(int n, const double* A, const double* b, double* x) {                                    \
-    RT a[orc::NPMAX * orc::NPMAX], bb[orc::NPMAX], xx[orc::NPMAX];                                             \
+    RT a[orc::NPMAX * orc::NPMAX] = {}, bb[orc::NPMAX] = {}, xx[orc::NPMAX] = {};                                           \
     for (int i = 0; i < n * n; ++i) a[i] = (RT)A[i];                                                           \
     for (int i = 0; i < n; ++i) bb[i] = (RT)b[i];                                                              \
     orc::lu_solve<RT>(n, a, bb, xx);                                                                           \
