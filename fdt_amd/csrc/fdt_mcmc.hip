@@ -9,7 +9,10 @@
 // volumes l, l+16, l+32, ... (VPL volumes per lane) and keeps, in VGPRs,
 //   the residual r = y - S0*M (Gaussian) or the data y (Rician),
 //   the mixture M = sum_k f_k s_k + (1 - f0 - sum_k f_k) iso + f0,
-//   the isotropic signal iso, the stick signals s_k and b*(g.x_k)^2.
+//   the isotropic signal iso and the stick signals s_k;
+// b*(g.x_k)^2, read only by the diffusivity proposals, sits in LDS (one block
+// per wave, lane-interleaved), so that K=3 at 18 volumes per lane fits two
+// waves per SIMD.
 // A proposal evaluates only what it changes; its likelihood change is
 // summed over the row with four DPP adds.  For Gaussian noise the change of
 // the sum of squares is accumulated as sum dr*(2r+dr) so that the energy
@@ -88,9 +91,26 @@ FDT_DEV float ell_of(float y, float p, float tau) {
 }
 
 FDT_DEV float g_term(float r, float dr) { return dr * fmaf(2.0f, r, dr); }
+// Compiler-only memory barrier: keeps the acquisition table in LDS (re-read
+// per proposal) instead of letting the loop-invariant loads be hoisted into
+// ~4*VPL VGPRs for the whole chain.
+FDT_DEV void reload_acq() { asm volatile("" ::: "memory"); }
 FDT_DEV int acc1(int c) { return c + 1; }
 FDT_DEV int rej1(int c) { return c + (1 << 16); }
 
+// compile-time loop: f(integral_constant<int, i>) for i = 0..N-1
+template <int N, class F>
+FDT_DEV void sfor(F&& f) {
+  [&]<int... is>(std::integer_sequence<int, is...>) { (f(std::integral_constant<int, is>{}), ...); }(
+      std::make_integer_sequence<int, N>{});
+}
+
+// One chain per 16-lane row.  Volume state lives in registers per lane;
+// everything that belongs to a proposal slot -- the parameter value, its
+// proposal width, its acceptance counters (acc + rej << 16) and its current
+// prior energy -- lives in the lane of that slot (sval/sprop/scnt/sprior) and
+// is read by a row broadcast.  Slots: f0 0, tau 1, d 2, d_std 3, R 4, S0 5,
+// theta_k 6+3k, phi_k 7+3k, f_k 8+3k (the theta prior sits in the theta slot).
 template <int K, int VPL, bool RIC>
 struct Chain {
   // lane-resident volume state, volume i = rl + 16 t
@@ -98,14 +118,10 @@ struct Chain {
   float M[VPL];
   float iso[VPL];
   float s[K][VPL];
-  float ba[K][VPL];
+  float* bal;  // b*(g.x_k)^2 of the lane's volumes, in the wave's LDS block: bal[(k*VPL+t)*64]
   float ell[RIC ? VPL : 1];
-  // row-uniform state
-  float S0, d, dstd, Rr, f0, tau;
-  float th[K], ph[K], f[K];
-  float pS0, pd, pdstd, pR, pf0, ptau, pth[K], pph[K], pf[K];
-  int cS0, cd, cdstd, cR, cf0, ctau, cth[K], cph[K], cf[K];  // acc + (rej << 16)
-  float prd, prdstd, prR, prf0, prth[K], prf[K];
+  float sval, sprop, sprior;
+  int scnt;
   double like;  // Gaussian: SSR; Rician: sum of ell
   bool last_ok;  // the lane's last slot holds a real volume
   int rl;
@@ -116,12 +132,30 @@ struct Chain {
   float fudge, halfN, Nf, Rmean, Rstd, Rfudge;
   const float4* acq;
 
-  FDT_DEV float mask_last(int t, float v) const { return (t == VPL - 1 && !last_ok) ? 0.0f : v; }
-  FDT_DEV float fsum() const {
-    float a = f0;
-#pragma unroll
-    for (int k = 0; k < K; ++k) a += f[k];
-    return a;
+  static constexpr int TH(int k) { return SL_FIB + 3 * k; }
+  static constexpr int PH(int k) { return SL_FIB + 3 * k + 1; }
+  static constexpr int FK(int k) { return SL_FIB + 3 * k + 2; }
+  template <int SL> FDT_DEV float v() const { return row_bcast<SL>(sval); }
+  template <int SL> FDT_DEV float width() const { return row_bcast<SL>(sprop); }
+  template <int SL> FDT_DEV float prior() const { return row_bcast<SL>(sprior); }
+  template <int SL> FDT_DEV void reject() { if (rl == SL) scnt += 1 << 16; }
+  template <int SL> FDT_DEV void accept(float val, float prn) {
+    if (rl == SL) { sval = val; scnt += 1; sprior = prn; }
+  }
+
+  FDT_DEV float ba(int k, int t) const { return bal[(k * VPL + t) * 64]; }
+  FDT_DEV float mask_last(int t, float x) const { return (t == VPL - 1 && !last_ok) ? 0.0f : x; }
+  // the fractions of the row: fk[k] and f0, their sum
+  struct Fr {
+    float fk[K];
+    float f0, sum;
+  };
+  FDT_DEV Fr fractions() const {
+    Fr q;
+    q.f0 = v<SL_F0>();
+    q.sum = q.f0;
+    sfor<K>([&](auto kc) { q.fk[kc] = v<FK(kc)>(); q.sum += q.fk[kc]; });
+    return q;
   }
   // energy change of the likelihood from a row-summed change `dl`
   FDT_DEV float dlik(float dl) const {
@@ -133,11 +167,11 @@ struct Chain {
       return halfN * lp;
     }
   }
-  FDT_DEV float theta_prior(float t) const {  // fibre.h:283-290
+  FDT_DEV static float theta_prior(float t) {  // fibre.h:283-290
     return t == 0.0f ? 0.0f : -flog(fabsf(__sinf(t)) * 0.5f);
   }
-  // R prior (fibre.h:816-843); returns false when outside its support
-  FDT_DEV bool r_prior(float Rv, float& pr) const {
+  // R prior (fibre.h:816-843) at diffusivity d; false outside its support
+  FDT_DEV bool r_prior(float Rv, float d, float& pr) const {
     float upper = 2.0f * Rmean, lower = Rmean - 2.0f * Rstd;
     if (Rmean > 0.5f) upper = 1.0f;
     if (lower < 0.0f) lower = 1e-8f;
@@ -150,16 +184,22 @@ struct Chain {
     pr = (Rv - Rmean) * (Rv - Rmean) / (Rstd * Rstd);
     return true;
   }
+  // mixture of slot t from the stored signals
+  FDT_DEV float mix(int t, const Fr& q) const {
+    float m = fmaf(1.0f - q.sum, iso[t], q.f0);
+#pragma unroll
+    for (int k = 0; k < K; ++k) m = fmaf(q.fk[k], s[k][t], m);
+    return m;
+  }
 
-  // rebuild M and the residual from the stored signals
+  // rebuild M and the residual (or the Rician terms) from the stored signals
   FDT_DEV void rebuild(const float* y) {
-    float fs = fsum();
+    const Fr q = fractions();
+    const float S0 = v<SL_S0>(), tau = v<SL_TAU>();
     double ss = 0.0;
 #pragma unroll
     for (int t = 0; t < VPL; ++t) {
-      float m = fmaf(1.0f - fs, iso[t], f0);
-#pragma unroll
-      for (int k = 0; k < K; ++k) m = fmaf(f[k], s[k][t], m);
+      float m = mix(t, q);
       M[t] = m;
       if constexpr (RIC) {
         ell[t] = ell_of(r[t], S0 * m, tau);
@@ -173,14 +213,16 @@ struct Chain {
   }
 
   // ---------------------------------------------------------- proposals
-  // Each returns through `acc` whether the move was accepted.
+  // n: standard normal draw, lu: log of the uniform draw of the slot.
   // f0 (CUDA/runmcmc_kernels.cu:842-888)
   FDT_DEV void prop_f0(float n, float lu) {
-    float f0n = f0 + n * pf0;
-    float fsn = fsum() - f0 + f0n;
-    if (f0n <= 0.0f || f0n >= 1.0f || fsn > 1.0f) { cf0 = rej1(cf0); return; }
-    float prn = ardf0 ? flog(f0n) : 0.0f;
-    float df = f0n - f0;
+    const float f0 = v<SL_F0>();
+    const float f0n = f0 + n * width<SL_F0>();
+    const Fr q = fractions();
+    const float fsn = q.sum - f0 + f0n;
+    if (f0n <= 0.0f || f0n >= 1.0f || fsn > 1.0f) { reject<SL_F0>(); return; }
+    const float prn = ardf0 ? flog(f0n) : 0.0f;
+    const float df = f0n - f0, S0 = v<SL_S0>(), tau = v<SL_TAU>();
     float a = 0.0f, en[RIC ? VPL : 1];
 #pragma unroll
     for (int t = 0; t < VPL; ++t) {
@@ -189,7 +231,7 @@ struct Chain {
       else a += mask_last(t, g_term(r[t], -S0 * dM));
     }
     a = row_sum16(a);
-    if (dlik(a) + (prn - prf0) < -lu) {
+    if (dlik(a) + (prn - prior<SL_F0>()) < -lu) {
 #pragma unroll
       for (int t = 0; t < VPL; ++t) {
         float dM = df * (1.0f - iso[t]);
@@ -197,14 +239,16 @@ struct Chain {
         if constexpr (RIC) ell[t] = en[t]; else r[t] -= S0 * dM;
       }
       like += (double)a;
-      f0 = f0n; prf0 = prn; cf0 = acc1(cf0);
-    } else cf0 = rej1(cf0);
+      accept<SL_F0>(f0n, prn);
+    } else reject<SL_F0>();
   }
   // tau (:890-922), Rician only
   FDT_DEV void prop_tau(float n, float lu) {
     if constexpr (RIC) {
-      float tn = tau + n * ptau;
-      if (tn <= 0.0f) { ctau = rej1(ctau); return; }
+      const float tau = v<SL_TAU>();
+      const float tn = tau + n * width<SL_TAU>();
+      if (tn <= 0.0f) { reject<SL_TAU>(); return; }
+      const float S0 = v<SL_S0>();
       float a = 0.0f, en[VPL];
 #pragma unroll
       for (int t = 0; t < VPL; ++t) { en[t] = ell_of(r[t], S0 * M[t], tn); a += mask_last(t, en[t] - ell[t]); }
@@ -214,15 +258,16 @@ struct Chain {
 #pragma unroll
         for (int t = 0; t < VPL; ++t) ell[t] = en[t];
         like += (double)a;
-        tau = tn; ctau = acc1(ctau);
-      } else ctau = rej1(ctau);
+        accept<SL_TAU>(tn, 0.0f);
+      } else reject<SL_TAU>();
     }
   }
   // S0 (:1125-1155)
   FDT_DEV void prop_S0(float n, float lu) {
-    float Sn = S0 + n * pS0;
-    if (Sn < 0.0f) { cS0 = rej1(cS0); return; }
-    float dS = Sn - S0;
+    const float S0 = v<SL_S0>();
+    const float Sn = S0 + n * width<SL_S0>();
+    if (Sn < 0.0f) { reject<SL_S0>(); return; }
+    const float dS = Sn - S0, tau = v<SL_TAU>();
     float a = 0.0f, en[RIC ? VPL : 1];
 #pragma unroll
     for (int t = 0; t < VPL; ++t) {
@@ -236,141 +281,166 @@ struct Chain {
         if constexpr (RIC) ell[t] = en[t]; else r[t] -= dS * M[t];
       }
       like += (double)a;
-      S0 = Sn; cS0 = acc1(cS0);
-    } else cS0 = rej1(cS0);
+      accept<SL_S0>(Sn, 0.0f);
+    } else reject<SL_S0>();
   }
-  // all stick signals and/or iso change: d (:924-985), d_std (:987-1051), R (:1053-1122)
-  FDT_DEV void prop_global(int which, float n, float lu) {
-    float dn = d, dsn = dstd, Rn = Rr, prn = 0.0f, prold = 0.0f;
+  // candidate mixture of slot t when the stick signals (NA) and/or the
+  // isotropic signal (NI) are re-evaluated with coefficients ca / ci
+  template <int SA, int SI, bool NA, bool NI>
+  FDT_DEV float mix_new(int t, const Fr& q, const SigC& ca, const SigC& ci) const {
+    float b = acq[rl + 16 * t].w;
+    float isn = NI ? sig<SI>(ci, b, b) : iso[t];
+    float m = fmaf(1.0f - q.sum, isn, q.f0);
+#pragma unroll
+    for (int k = 0; k < K; ++k) m = fmaf(q.fk[k], NA ? sig<SA>(ca, ba(k, t), b) : s[k][t], m);
+    return m;
+  }
+  template <int SA, int SI, bool NA, bool NI>
+  FDT_DEV void store_new(int t, const SigC& ca, const SigC& ci) {
+    float b = acq[rl + 16 * t].w;
+    if (NI) iso[t] = sig<SI>(ci, b, b);
+    if (NA) {
+#pragma unroll
+      for (int k = 0; k < K; ++k) s[k][t] = sig<SA>(ca, ba(k, t), b);
+    }
+  }
+  // dispatch over the coefficient forms that occur: model 1 (0,0); model 2
+  // (1,1) or (0,0) below d_std 1e-5; model 3 sticks 2 with iso 1 or 0
+  template <class F>
+  FDT_DEV void by_forms(const SigC& ca, const SigC& ci, bool na, bool ni, F&& f) const {
+    using I0 = std::integral_constant<int, 0>;
+    using I1 = std::integral_constant<int, 1>;
+    using I2 = std::integral_constant<int, 2>;
+    using T_ = std::true_type;
+    using F_ = std::false_type;
+    if (na && ni) {
+      if (ca.kind == 0) f(I0{}, I0{}, T_{}, T_{});
+      else if (ca.kind == 1) f(I1{}, I1{}, T_{}, T_{});
+      else if (ci.kind == 1) f(I2{}, I1{}, T_{}, T_{});
+      else f(I2{}, I0{}, T_{}, T_{});
+    } else if (ni) {
+      if (ci.kind == 1) f(I0{}, I1{}, F_{}, T_{});
+      else f(I0{}, I0{}, F_{}, T_{});
+    } else {
+      f(I2{}, I0{}, T_{}, F_{});
+    }
+  }
+  // all stick signals and/or iso change: d (:924-985), d_std (:987-1051), R
+  // (:1053-1122).  Neither the candidate signals nor the candidate mixture are
+  // kept: the row sum is accumulated on the fly and everything is re-evaluated
+  // when the move is accepted, which keeps the live registers to the chain
+  // state.
+  template <int SL>
+  FDT_DEV void prop_global(float n, float lu) {
+    const float d = v<SL_D>(), dstd = v<SL_DSTD>(), Rr = v<SL_R>();
+    float dn = d, dsn = dstd, Rn = Rr, prn = 0.0f, xn;
     bool new_aniso = true, new_iso = true;
-    if (which == SL_D) {
-      dn = d + n * pd;
-      if (dn < 0.0f || dn > UPPERDIFF_F) { cd = rej1(cd); return; }
+    if constexpr (SL == SL_D) {
+      dn = xn = d + n * width<SL>();
+      if (dn < 0.0f || dn > UPPERDIFF_F) { reject<SL>(); return; }
       prn = model == 3 ? (1.0f - 3.0f) * flog(dn) + 4000.0f * dn : 0.0f;
-      prold = prd;
-    } else if (which == SL_DSTD) {
-      dsn = dstd + n * pdstd;
+    } else if constexpr (SL == SL_DSTD) {
+      dsn = xn = dstd + n * width<SL>();
       float up = model == 3 ? 0.004f : 0.01f;
-      if (dsn <= 0.0f || dsn > up) { cdstd = rej1(cdstd); return; }
+      if (dsn <= 0.0f || dsn > up) { reject<SL>(); return; }
       prn = flog(dsn);
-      prold = prdstd;
       new_aniso = (model == 2);
     } else {
-      Rn = Rr + n * pR;
-      if (!r_prior(Rn, prn)) { cR = rej1(cR); return; }
-      prold = prR;
+      Rn = xn = Rr + n * width<SL>();
+      if (!r_prior(Rn, d, prn)) { reject<SL>(); return; }
       new_iso = false;
     }
-    SigC ca = aniso_coef(model, dn, dsn, Rn), ci = iso_coef(model, dn, dsn);
-    float sn[K][VPL], isn[VPL], Mn[VPL], en[RIC ? VPL : 1];
-    float fs = fsum();
+    const SigC ca = aniso_coef(model, dn, dsn, Rn), ci = iso_coef(model, dn, dsn);
+    const Fr q = fractions();
+    const float S0 = v<SL_S0>(), tau = v<SL_TAU>();
+    reload_acq();
     float a = 0.0f;
-    if (new_aniso) {
-      by_kind(ca.kind, [&](auto SK) {
-#pragma unroll
-        for (int t = 0; t < VPL; ++t) {
-          float b = acq[rl + 16 * t].w;
-#pragma unroll
-          for (int k = 0; k < K; ++k) sn[k][t] = sig<decltype(SK)::value>(ca, ba[k][t], b);
-        }
-      });
-    } else {
-#pragma unroll
-      for (int t = 0; t < VPL; ++t)
-#pragma unroll
-        for (int k = 0; k < K; ++k) sn[k][t] = s[k][t];
-    }
-    if (new_iso) {
-      by_kind(ci.kind, [&](auto SK) {
-#pragma unroll
-        for (int t = 0; t < VPL; ++t) {
-          float b = acq[rl + 16 * t].w;
-          isn[t] = sig<decltype(SK)::value>(ci, b, b);
-        }
-      });
-    } else {
-#pragma unroll
-      for (int t = 0; t < VPL; ++t) isn[t] = iso[t];
-    }
-#pragma unroll
-    for (int t = 0; t < VPL; ++t) {
-      float m = fmaf(1.0f - fs, isn[t], f0);
-#pragma unroll
-      for (int k = 0; k < K; ++k) m = fmaf(f[k], sn[k][t], m);
-      Mn[t] = m;
-      if constexpr (RIC) { en[t] = ell_of(r[t], S0 * m, tau); a += mask_last(t, en[t] - ell[t]); }
-      else a += mask_last(t, g_term(r[t], -S0 * (m - M[t])));
-    }
-    a = row_sum16(a);
-    bool ok = dlik(a) + (prn - prold) < -lu;
-    if (ok) {
+    by_forms(ca, ci, new_aniso, new_iso, [&](auto SA, auto SI, auto NA, auto NI) {
 #pragma unroll
       for (int t = 0; t < VPL; ++t) {
-#pragma unroll
-        for (int k = 0; k < K; ++k) s[k][t] = sn[k][t];
-        iso[t] = isn[t];
-        if constexpr (RIC) ell[t] = en[t]; else r[t] -= S0 * (Mn[t] - M[t]);
-        M[t] = Mn[t];
+        float mn = mix_new<decltype(SA)::value, decltype(SI)::value, decltype(NA)::value, decltype(NI)::value>(t, q, ca, ci);
+        if constexpr (RIC) a += mask_last(t, ell_of(r[t], S0 * mn, tau) - ell[t]);
+        else a += mask_last(t, g_term(r[t], -S0 * (mn - M[t])));
       }
+    });
+    a = row_sum16(a);
+    if (dlik(a) + (prn - prior<SL>()) < -lu) {
+      // store the new signals, then form the mixture from them with the
+      // same expression as the candidate (bit-identical to it)
+      reload_acq();
+      by_forms(ca, ci, new_aniso, new_iso, [&](auto SA, auto SI, auto NA, auto NI) {
+#pragma unroll
+        for (int t = 0; t < VPL; ++t) {
+          store_new<decltype(SA)::value, decltype(SI)::value, decltype(NA)::value, decltype(NI)::value>(t, ca, ci);
+          float mn = mix(t, q);
+          if constexpr (RIC) ell[t] = ell_of(r[t], S0 * mn, tau); else r[t] -= S0 * (mn - M[t]);
+          M[t] = mn;
+        }
+      });
       like += (double)a;
-    }
-    if (which == SL_D) { if (ok) { d = dn; prd = prn; cd = acc1(cd); } else cd = rej1(cd); }
-    else if (which == SL_DSTD) { if (ok) { dstd = dsn; prdstd = prn; cdstd = acc1(cdstd); } else cdstd = rej1(cdstd); }
-    else { if (ok) { Rr = Rn; prR = prn; cR = acc1(cR); } else cR = rej1(cR); }
+      accept<SL>(xn, prn);
+    } else reject<SL>();
   }
-  // theta_k / phi_k (:1157-1265)
-  template <int k>
-  FDT_DEV void prop_angle(bool is_th, float n, float lu) {
-    float tn = th[k], pn = ph[k], prn = prth[k];
-    if (is_th) { tn = th[k] + n * pth[k]; prn = theta_prior(tn); }
-    else pn = ph[k] + n * pph[k];
+  // theta_k / phi_k (:1157-1265); the theta prior lives in the theta slot
+  template <int k, bool IS_TH>
+  FDT_DEV void prop_angle(float n, float lu) {
+    constexpr int SL = IS_TH ? TH(k) : PH(k);
+    float tn = v<TH(k)>(), pn = v<PH(k)>(), prn = 0.0f, prold = 0.0f, xn;
+    if constexpr (IS_TH) { tn = xn = tn + n * width<SL>(); prn = theta_prior(tn); prold = prior<SL>(); }
+    else pn = xn = pn + n * width<SL>();
     float st, ct, sp, cp;
     __sincosf(tn, &st, &ct);
     __sincosf(pn, &sp, &cp);
-    float x0 = st * cp, x1 = st * sp, x2 = ct;
-    SigC ca = aniso_coef(model, d, dstd, Rr);
-    float ban[VPL], sn[VPL], en[RIC ? VPL : 1];
+    const float x0 = st * cp, x1 = st * sp, x2 = ct;
+    const SigC ca = aniso_coef(model, v<SL_D>(), v<SL_DSTD>(), v<SL_R>());
+    const float fk = v<FK(k)>(), S0 = v<SL_S0>(), tau = v<SL_TAU>();
+    reload_acq();
+    float sn[VPL], en[RIC ? VPL : 1];
     float a = 0.0f;
     by_kind(ca.kind, [&](auto SK) {
 #pragma unroll
       for (int t = 0; t < VPL; ++t) {
-        float4 q = acq[rl + 16 * t];
-        float dp = fmaf(q.x, x0, fmaf(q.y, x1, q.z * x2));
-        ban[t] = q.w * dp * dp;
-        sn[t] = sig<decltype(SK)::value>(ca, ban[t], q.w);
+        float4 g = acq[rl + 16 * t];
+        float dp = fmaf(g.x, x0, fmaf(g.y, x1, g.z * x2));
+        sn[t] = sig<decltype(SK)::value>(ca, g.w * dp * dp, g.w);
       }
     });
 #pragma unroll
     for (int t = 0; t < VPL; ++t) {
-      float dM = f[k] * (sn[t] - s[k][t]);
+      float dM = fk * (sn[t] - s[k][t]);
       if constexpr (RIC) { en[t] = ell_of(r[t], S0 * (M[t] + dM), tau); a += mask_last(t, en[t] - ell[t]); }
       else a += mask_last(t, g_term(r[t], -S0 * dM));
     }
     a = row_sum16(a);
-    if (dlik(a) + (prn - prth[k]) < -lu) {
+    if (dlik(a) + (prn - prold) < -lu) {
+      reload_acq();
 #pragma unroll
       for (int t = 0; t < VPL; ++t) {
-        float dM = f[k] * (sn[t] - s[k][t]);
+        float dM = fk * (sn[t] - s[k][t]);
         M[t] += dM;
         if constexpr (RIC) ell[t] = en[t]; else r[t] -= S0 * dM;
         s[k][t] = sn[t];
-        ba[k][t] = ban[t];
+        // b*(g.x)^2 of the accepted direction, recomputed rather than kept
+        // live through the row sum (the same expression as above)
+        float4 g = acq[rl + 16 * t];
+        float dp = fmaf(g.x, x0, fmaf(g.y, x1, g.z * x2));
+        bal[(k * VPL + t) * 64] = g.w * dp * dp;
       }
       like += (double)a;
-      th[k] = tn; ph[k] = pn; prth[k] = prn;
-      if (is_th) cth[k] = acc1(cth[k]); else cph[k] = acc1(cph[k]);
-    } else {
-      if (is_th) cth[k] = rej1(cth[k]); else cph[k] = rej1(cph[k]);
-    }
+      if constexpr (IS_TH) accept<SL>(xn, prn);
+      else if (rl == SL) { sval = xn; scnt += 1; }
+    } else reject<SL>();
   }
   // f_k (:1267-1322)
   template <int k>
   FDT_DEV void prop_f(float n, float lu) {
-    float fn = f[k] + n * pf[k];
-    float fsn = fsum() - f[k] + fn;
-    if (fn <= 0.0f || fn >= 1.0f || fsn > 1.0f) { cf[k] = rej1(cf[k]); return; }
-    float prn = (can_ard && lam[k]) ? fudge * flog(fn) : 0.0f;
-    float df = fn - f[k];
+    constexpr int SL = FK(k);
+    const Fr q = fractions();
+    const float fn = q.fk[k] + n * width<SL>();
+    const float fsn = q.sum - q.fk[k] + fn;
+    if (fn <= 0.0f || fn >= 1.0f || fsn > 1.0f) { reject<SL>(); return; }
+    const float prn = (can_ard && lam[k]) ? fudge * flog(fn) : 0.0f;
+    const float df = fn - q.fk[k], S0 = v<SL_S0>(), tau = v<SL_TAU>();
     float a = 0.0f, en[RIC ? VPL : 1];
 #pragma unroll
     for (int t = 0; t < VPL; ++t) {
@@ -379,7 +449,7 @@ struct Chain {
       else a += mask_last(t, g_term(r[t], -S0 * dM));
     }
     a = row_sum16(a);
-    if (dlik(a) + (prn - prf[k]) < -lu) {
+    if (dlik(a) + (prn - prior<SL>()) < -lu) {
 #pragma unroll
       for (int t = 0; t < VPL; ++t) {
         float dM = df * (s[k][t] - iso[t]);
@@ -387,31 +457,26 @@ struct Chain {
         if constexpr (RIC) ell[t] = en[t]; else r[t] -= S0 * dM;
       }
       like += (double)a;
-      f[k] = fn; prf[k] = prn; cf[k] = acc1(cf[k]);
-    } else cf[k] = rej1(cf[k]);
+      accept<SL>(fn, prn);
+    } else reject<SL>();
   }
 
-  // proposal-width adaptation every upe jumps (:1341-1395)
-  FDT_DEV static void upd(float& p, int& c) {
-    float acc = (float)(c & 0xFFFF), rej = (float)(c >> 16);
-    p *= sqrtf((acc + 1.0f) / (rej + 1.0f));
-    p = fminf(p, MAXPROP_F);
-    c = 0;
-  }
-  FDT_DEV void update_proposals(bool ric) {
-    upd(pd, cd);
-    if (ric) upd(ptau, ctau);
-    if (incf0) upd(pf0, cf0);
-    if (model >= 2) { upd(pdstd, cdstd); if (model == 3) upd(pR, cR); }
-    upd(pS0, cS0);
-#pragma unroll
-    for (int k = 0; k < K; ++k) { upd(pth[k], cth[k]); upd(pph[k], cph[k]); upd(pf[k], cf[k]); }
+  // proposal-width adaptation every upe jumps (:1341-1395), all slots of the
+  // row in parallel; slots of parameters that are not sampled keep counters
+  // 0/0 and are never read
+  FDT_DEV void update_proposals() {
+    float acc = (float)(scnt & 0xFFFF), rej = (float)(scnt >> 16);
+    sprop *= sqrtf((acc + 1.0f) / (rej + 1.0f));
+    sprop = fminf(sprop, MAXPROP_F);
+    scnt = 0;
   }
 };
 
 template <int K, int VPL, bool RIC>
-__global__ __launch_bounds__(256) void mcmc_kernel(McmcArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VPL <= 18 ? 2 : 1, 2))) void mcmc_kernel(McmcArgs a) {
+  using C = Chain<K, VPL, RIC>;
   __shared__ float4 sacq[16 * VPL];
+  __shared__ float sba[4 * K * VPL * 64];
   for (int i = threadIdx.x; i < 16 * VPL; i += blockDim.x) sacq[i] = a.acq[i];
   __syncthreads();
   const int lane = __lane_id();
@@ -419,9 +484,10 @@ __global__ __launch_bounds__(256) void mcmc_kernel(McmcArgs a) {
   const int vox = blockIdx.x * 16 + (threadIdx.x >> 4);
   if (vox >= a.nvox) return;
 
-  Chain<K, VPL, RIC> c;
+  C c;
   c.rl = rl;
   c.acq = sacq;
+  c.bal = sba + (threadIdx.x >> 6) * (K * VPL * 64) + lane;
   c.model = a.model;
   c.incf0 = a.f0 != 0;
   c.ardf0 = a.ardf0 != 0;
@@ -431,6 +497,8 @@ __global__ __launch_bounds__(256) void mcmc_kernel(McmcArgs a) {
   c.Nf = (float)a.N;
   c.Rmean = a.Rmean; c.Rstd = a.Rstd; c.Rfudge = a.Rfudge;
   c.last_ok = (rl + 16 * (VPL - 1)) < a.N;
+#pragma unroll
+  for (int k = 0; k < K; ++k) c.lam[k] = (k == 0) ? (a.all_ard != 0) : (a.no_ard == 0);
 
   // ---- data (with the Rician non-positive replacement, CUDA/xfibres_gpu.cu:350-367)
   float y[VPL];
@@ -454,61 +522,77 @@ __global__ __launch_bounds__(256) void mcmc_kernel(McmcArgs a) {
     }
   }
 
-  // ---- chain initialisation (init_Fibres_Multifibres_kernel, :265-504)
+  // ---- chain initialisation (init_Fibres_Multifibres_kernel, :265-504):
+  // every lane loads the parameter of its slot and evaluates its prior
   const float* P = a.params + (size_t)vox * a.np;
   const int add = a.model >= 2 ? 1 : 0;
-  c.S0 = P[0];
-  c.d = P[1];
-  if (c.d < 0.0f || c.d > UPPERDIFF_F) c.d = 2e-3f;
+  float d = P[1];
+  if (d < 0.0f || d > UPPERDIFF_F) d = 2e-3f;
+  float dstd = 0.0f, Rr = 0.0f;
   if (a.model >= 2) {
-    c.dstd = P[2];
+    dstd = P[2];
     float up = a.model == 3 ? 0.004f : 0.01f;
-    if (c.dstd < 0.0f || c.dstd > up) c.dstd = c.d / 10.0f;
-    c.Rr = a.model == 3 ? a.Rmean : 0.0f;
-  } else { c.dstd = 0.0f; c.Rr = 0.0f; }
-  c.f0 = c.incf0 ? P[a.np - 1] : 0.0f;
-  c.tau = (RIC && a.tau) ? a.tau[vox] : 0.0f;
-#pragma unroll
-  for (int k = 0; k < K; ++k) {
-    c.f[k] = P[2 + 3 * k + add];
-    c.th[k] = P[3 + 3 * k + add];
-    c.ph[k] = P[4 + 3 * k + add];
-    c.pth[k] = c.pph[k] = c.pf[k] = 0.2f;
-    c.cth[k] = c.cph[k] = c.cf[k] = 0;
-    c.prth[k] = c.theta_prior(c.th[k]);
-    c.lam[k] = (k == 0) ? (a.all_ard != 0) : (a.no_ard == 0);
-    c.prf[k] = (c.f[k] <= 0.0f || c.f[k] >= 1.0f) ? 0.0f : a.fudge * (c.lam[k] ? flog(c.f[k]) : 0.0f);
+    if (dstd < 0.0f || dstd > up) dstd = d / 10.0f;
+    if (a.model == 3) Rr = a.Rmean;
   }
-  c.cS0 = c.cd = c.cdstd = c.cR = c.cf0 = c.ctau = 0;
-  c.prd = (a.model == 3 && c.d >= 0.0f && c.d <= UPPERDIFF_F) ? (1.0f - 3.0f) * flog(c.d) + 4000.0f * c.d : 0.0f;
-  c.prdstd = 0.0f;
-  c.prR = 0.0f;
-  if (a.model >= 2) {
+  float val = 0.0f, pr = 0.0f, wd = 0.2f;  // f0 and the fibre slots start at width 0.2 (:497-503)
+  if (rl == SL_F0) {
+    val = c.incf0 ? P[a.np - 1] : 0.0f;
+    if (c.incf0 && c.ardf0 && val > 0.0f && val < 1.0f) pr = flog(val);
+  } else if (rl == SL_TAU) {
+    val = (RIC && a.tau) ? a.tau[vox] : 0.0f;
+    wd = val / 2.0f;
+  } else if (rl == SL_D) {
+    val = d;
+    if (a.model == 3) pr = (1.0f - 3.0f) * flog(d) + 4000.0f * d;
+    wd = d / 10.0f;
+  } else if (rl == SL_DSTD) {
+    val = dstd;
     float up = a.model == 3 ? 0.004f : 0.01f;
-    if (c.dstd > 0.0f && c.dstd <= up) c.prdstd = flog(c.dstd);
-    if (a.model == 3) { float pr; if (c.r_prior(c.Rr, pr)) c.prR = pr; }
+    if (a.model >= 2 && dstd > 0.0f && dstd <= up) pr = flog(dstd);
+    wd = dstd / 10.0f;
+  } else if (rl == SL_R) {
+    val = Rr;
+    float q;
+    if (a.model == 3 && c.r_prior(Rr, d, q)) pr = q;
+    wd = Rr / 10.0f;
+  } else if (rl == SL_S0) {
+    val = P[0];
+    wd = val / 10.0f;
+  } else if (rl < SL_FIB + 3 * K) {
+    int k = (rl - SL_FIB) / 3, w = (rl - SL_FIB) % 3;
+    if (w == 0) { val = P[3 + 3 * k + add]; pr = C::theta_prior(val); }
+    else if (w == 1) val = P[4 + 3 * k + add];
+    else {
+      val = P[2 + 3 * k + add];
+      bool lk = (k == 0) ? (a.all_ard != 0) : (a.no_ard == 0);
+      pr = (val <= 0.0f || val >= 1.0f) ? 0.0f : a.fudge * (lk ? flog(val) : 0.0f);
+    }
   }
-  c.prf0 = (c.incf0 && c.ardf0 && c.f0 > 0.0f && c.f0 < 1.0f) ? flog(c.f0) : 0.0f;
+  c.sval = val;
+  c.sprior = pr;
+  c.sprop = wd;
+  c.scnt = 0;
   {
-    SigC ca = aniso_coef(a.model, c.d, c.dstd, c.Rr), ci = iso_coef(a.model, c.d, c.dstd);
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
+    SigC ca = aniso_coef(a.model, d, dstd, Rr), ci = iso_coef(a.model, d, dstd);
+    sfor<K>([&](auto kc) {
+      constexpr int k = kc;
       float st, ct, sp, cp;
-      __sincosf(c.th[k], &st, &ct);
-      __sincosf(c.ph[k], &sp, &cp);
+      __sincosf(c.template v<C::TH(k)>(), &st, &ct);
+      __sincosf(c.template v<C::PH(k)>(), &sp, &cp);
       float x0 = st * cp, x1 = st * sp, x2 = ct;
 #pragma unroll
       for (int t = 0; t < VPL; ++t) {
-        float4 q = sacq[rl + 16 * t];
-        float dp = fmaf(q.x, x0, fmaf(q.y, x1, q.z * x2));
-        c.ba[k][t] = q.w * dp * dp;
+        float4 g = sacq[rl + 16 * t];
+        float dp = fmaf(g.x, x0, fmaf(g.y, x1, g.z * x2));
+        c.bal[(k * VPL + t) * 64] = g.w * dp * dp;
       }
-    }
+    });
     by_kind(ca.kind, [&](auto SK) {
 #pragma unroll
       for (int t = 0; t < VPL; ++t)
 #pragma unroll
-        for (int k = 0; k < K; ++k) c.s[k][t] = sig<decltype(SK)::value>(ca, c.ba[k][t], sacq[rl + 16 * t].w);
+        for (int k = 0; k < K; ++k) c.s[k][t] = sig<decltype(SK)::value>(ca, c.ba(k, t), sacq[rl + 16 * t].w);
     });
     by_kind(ci.kind, [&](auto SK) {
 #pragma unroll
@@ -520,8 +604,22 @@ __global__ __launch_bounds__(256) void mcmc_kernel(McmcArgs a) {
     for (int t = 0; t < VPL; ++t) c.r[t] = y[t];
   }
   c.rebuild(y);
-  c.pS0 = c.S0 / 10.0f; c.pd = c.d / 10.0f; c.pdstd = c.dstd / 10.0f;
-  c.ptau = c.tau / 2.0f; c.pf0 = 0.2f; c.pR = c.Rr / 10.0f;
+
+  // record destination of the lane's slot (:1326-1339)
+  float* rec_base = nullptr;
+  size_t rec_stride = (size_t)a.nvox;  // per sample
+  if (rl == SL_F0) rec_base = a.f0s;
+  else if (rl == SL_TAU) rec_base = a.taus;
+  else if (rl == SL_D) rec_base = a.d;
+  else if (rl == SL_DSTD) rec_base = a.dstd;
+  else if (rl == SL_R) rec_base = a.R;
+  else if (rl == SL_S0) rec_base = a.S0;
+  else if (rl < SL_FIB + 3 * K) {
+    int k = (rl - SL_FIB) / 3, w = (rl - SL_FIB) % 3;
+    float* arr = w == 0 ? a.th : (w == 1 ? a.ph : a.f);
+    if (arr) rec_base = arr + (size_t)k * a.nsamples * a.nvox;
+  }
+  if (rec_base) rec_base += vox;
 
   const uint64_t gvox = a.vox_offset + (uint64_t)vox;
   const int total = a.nburn + a.njumps;
@@ -530,49 +628,32 @@ __global__ __launch_bounds__(256) void mcmc_kernel(McmcArgs a) {
     draw(a.seed, gvox, (uint32_t)j, (uint32_t)rl, nl, lul);
     if (c.incf0) c.prop_f0(row_bcast<SL_F0>(nl), row_bcast<SL_F0>(lul));
     if (RIC) c.prop_tau(row_bcast<SL_TAU>(nl), row_bcast<SL_TAU>(lul));
-    c.prop_global(SL_D, row_bcast<SL_D>(nl), row_bcast<SL_D>(lul));
+    c.template prop_global<SL_D>(row_bcast<SL_D>(nl), row_bcast<SL_D>(lul));
     if (a.model >= 2) {
-      c.prop_global(SL_DSTD, row_bcast<SL_DSTD>(nl), row_bcast<SL_DSTD>(lul));
-      if (a.model == 3) c.prop_global(SL_R, row_bcast<SL_R>(nl), row_bcast<SL_R>(lul));
+      c.template prop_global<SL_DSTD>(row_bcast<SL_DSTD>(nl), row_bcast<SL_DSTD>(lul));
+      if (a.model == 3) c.template prop_global<SL_R>(row_bcast<SL_R>(nl), row_bcast<SL_R>(lul));
     }
     c.prop_S0(row_bcast<SL_S0>(nl), row_bcast<SL_S0>(lul));
-    [&]<int... ks>(std::integer_sequence<int, ks...>) {
-      ((c.template prop_angle<ks>(true, row_bcast<SL_FIB + 3 * ks>(nl), row_bcast<SL_FIB + 3 * ks>(lul)),
-        c.template prop_angle<ks>(false, row_bcast<SL_FIB + 3 * ks + 1>(nl), row_bcast<SL_FIB + 3 * ks + 1>(lul)),
-        c.template prop_f<ks>(row_bcast<SL_FIB + 3 * ks + 2>(nl), row_bcast<SL_FIB + 3 * ks + 2>(lul))), ...);
-    }(std::make_integer_sequence<int, K>{});
+    sfor<K>([&](auto kc) {
+      constexpr int k = kc;
+      c.template prop_angle<k, true>(row_bcast<C::TH(k)>(nl), row_bcast<C::TH(k)>(lul));
+      c.template prop_angle<k, false>(row_bcast<C::PH(k)>(nl), row_bcast<C::PH(k)>(lul));
+      c.template prop_f<k>(row_bcast<C::FK(k)>(nl), row_bcast<C::FK(k)>(lul));
+    });
 
-    // record (:1326-1339): lane q of the row writes quantity q
+    // record: the lane of each sampled slot writes its value
     if (j >= a.nburn) {
       int rec = j - a.nburn + 1;
       if (a.se > 0 && rec % a.se == 0) {
         int smp = rec / a.se - 1;
-        if (smp < a.nsamples) {
-          size_t sv = (size_t)smp * a.nvox + vox;
-          float val = 0.0f;
-          float* dst = nullptr;
-#pragma unroll
-          for (int k = 0; k < K; ++k) {
-            size_t kv = ((size_t)k * a.nsamples + smp) * a.nvox + vox;
-            if (rl == k) { val = c.th[k]; dst = a.th + kv; }
-            if (rl == K + k) { val = c.ph[k]; dst = a.ph + kv; }
-            if (rl == 2 * K + k) { val = c.f[k]; dst = a.f + kv; }
-          }
-          if (rl == 3 * K) { val = c.S0; dst = a.S0 + sv; }
-          if (rl == 3 * K + 1) { val = c.d; dst = a.d + sv; }
-          if (rl == 3 * K + 2 && a.dstd) { val = c.dstd; dst = a.dstd + sv; }
-          if (rl == 3 * K + 3 && a.R) { val = c.Rr; dst = a.R + sv; }
-          if (rl == 3 * K + 4 && a.f0s) { val = c.f0; dst = a.f0s + sv; }
-          if (rl == 3 * K + 5 && a.taus) { val = c.tau; dst = a.taus + sv; }
-          if (dst) *dst = val;
-        }
+        if (smp < a.nsamples && rec_base) rec_base[(size_t)smp * rec_stride] = c.sval;
       }
     }
-    if ((j + 1) % a.upe == 0) c.update_proposals(RIC);
+    if ((j + 1) % a.upe == 0) c.update_proposals();
   }
-  if (a.energy && rl == 0) {
-    float e = RIC ? (float)(-(double)a.N * (double)flog(c.tau) - c.like) : (float)c.like;
-    a.energy[vox] = e;
+  if (a.energy) {
+    float tau = c.template v<SL_TAU>();
+    if (rl == 0) a.energy[vox] = RIC ? (float)(-(double)a.N * (double)flog(tau) - c.like) : (float)c.like;
   }
 }
 
