@@ -9,9 +9,11 @@ diffusivities), nf=3, 18 b=0 + 90 directions at each of b=1000/2000/3000
 SNR 25, synthetic (seeded) data.
 
 Also reported: the roofline position of the MCMC kernel (algorithmic FP32
-flops per launch / its HIP-event duration, against 157.3 TFLOP/s) and the
-CPU baseline: the oracle's FP32 restatement of the reference CPU xfibres,
-timed on this host on a bounded sample of the same voxels.
+flops per launch / its HIP-event duration, against 157.3 TFLOP/s), the
+end-to-end rate from host buffers (upload + run + download of all outputs),
+and the CPU baseline: the oracle's FP32 restatement of the reference CPU
+xfibres, timed on this host on a bounded sample of the same voxels, on all
+threads and on one.
 """
 from __future__ import annotations
 
@@ -27,6 +29,10 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
 
 FP32_PEAK_TFLOPS = 157.3
+# HBM bytes per launch of mcmc_kernel<3,18,false> on the full 250,968-voxel
+# workload: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE, each scaled by the same
+# counter's reading for a copy kernel of known size (profiles/mcmc_c1_traffic.md)
+MCMC_TRAFFIC_BYTES = None
 METRIC = "voxels/s, 3-fibre multi-shell bedpostx; surrogate vs plain at equal dyad error"
 
 
@@ -51,14 +57,10 @@ def flops_per_jump(model: int, K: int, N: int, f0: bool = False, rician: bool = 
     return N * (P * L + R)
 
 
-def cpu_baseline(v, opts, seconds=15.0, threads=None):
-    from oracle import oracle as O
-    threads = threads or min(16, os.cpu_count() or 1)
+def _cpu_rate(O, o, v, threads, seconds, rng, start_n):
+    """voxels/s of the FP32 CPU restatement (LM + full chain per voxel) on `threads` threads"""
     O.set_threads(threads)
-    o = O.opts(nfibres=opts["nfibres"], model=opts["model"], nburn=opts["nburn"], njumps=opts["njumps"],
-               sampleevery=opts["sampleevery"])
-    rng = np.random.default_rng(12345)
-    n = max(threads, 16)
+    n = start_n
     done, t_used = 0, 0.0
     while t_used < seconds and done < 4096:
         idx = rng.choice(v["data"].shape[0], n, replace=False)
@@ -69,10 +71,21 @@ def cpu_baseline(v, opts, seconds=15.0, threads=None):
         t_used += time.perf_counter() - t
         done += n
         n = min(2 * n, 4096 - done) if t_used < seconds / 3 else n
-    return dict(value=done / t_used, unit="voxels/s", cores=threads, kind="port",
+    return done / t_used, done, t_used
+
+
+def cpu_baseline(v, opts, seconds=15.0, threads=None):
+    from oracle import oracle as O
+    threads = threads or min(16, os.cpu_count() or 1)
+    o = O.opts(nfibres=opts["nfibres"], model=opts["model"], nburn=opts["nburn"], njumps=opts["njumps"],
+               sampleevery=opts["sampleevery"])
+    rng = np.random.default_rng(12345)
+    rate, done, t_used = _cpu_rate(O, o, v, threads, seconds, rng, max(threads, 16))
+    rate1, done1, t1 = _cpu_rate(O, o, v, 1, seconds / 3, rng, 2)
+    return dict(value=rate, unit="voxels/s", cores=threads, kind="port", value_1core=rate1,
                 sample=f"{done} voxels drawn at random from the workload, full LM + {opts['nburn']}+{opts['njumps']} "
                        f"MCMC sweeps each, oracle FP32 restatement of xfibres on {threads} OpenMP threads "
-                       f"({t_used:.1f} s of CPU work)")
+                       f"({t_used:.1f} s of CPU work); value_1core: {done1} voxels on 1 thread ({t1:.1f} s)")
 
 
 def main():
@@ -118,9 +131,21 @@ def main():
     tm = x.timing()
     x.close()
 
+    # end to end once more, from host buffers: upload, run, download of every output
+    x2 = F.Xfibres(**opts)
+    x2.set_acquisition(v["bvals"], v["bvecs"])
+    x2.sync()
+    t = time.perf_counter()
+    x2.load_data(v["data"])
+    x2.run(0)
+    x2.outputs()
+    e2e = nvox / (time.perf_counter() - t)
+    x2.close()
+
     fl_launch = flops_per_jump(2, 3, N) * (opts["nburn"] + opts["njumps"]) * nvox
     mc_ms = float(np.mean(ms_mcmc))
     achieved = fl_launch / (mc_ms / 1e3) / 1e12
+    traffic = MCMC_TRAFFIC_BYTES if nvox == synth.C1_NVOX else None
     line = {
         "metric": METRIC, "value": value, "unit": "voxels/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
@@ -131,10 +156,13 @@ def main():
                    "init": "plain LM (tensor start -> PVM_single -> PVM_multi)",
                    "ms_fit": float(np.mean(ms_fit)), "ms_mcmc": mc_ms, "ms_finish": float(np.mean(ms_fin)),
                    "mcmc_tile": f"{tm['lanes']} lanes x {tm['vpl']} volumes per voxel"},
-        "roofline": {"bound": "valu-fp32", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None,
-                     "kernel": "mcmc_kernel<3,18,false>",
-                     "flops_per_launch": fl_launch},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP32_PEAK_TFLOPS, "traffic": traffic,
+                     "kernel": "mcmc_kernel<3,18,false>", "flops_per_launch": fl_launch,
+                     "compute": "FP32 vector ALU (the path has no matrix-shaped work); peak is the published FP32 "
+                                "figure, which counts packed FP32 -- this build disables packed FP32, so half of it "
+                                "is the reachable issue ceiling"},
+        "end_to_end_voxels_per_s": e2e,
     }
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(v, opts, seconds=args.cpu_seconds)

@@ -21,12 +21,17 @@
 using namespace fdt;
 
 extern "C" {
-hipError_t fdt_launch_fit_k1_0(const FitArgs*, hipStream_t);
-hipError_t fdt_launch_fit_k2_0(const FitArgs*, hipStream_t);
-hipError_t fdt_launch_fit_k3_0(const FitArgs*, hipStream_t);
-hipError_t fdt_launch_fit_k1_1(const FitArgs*, hipStream_t);
-hipError_t fdt_launch_fit_k2_1(const FitArgs*, hipStream_t);
-hipError_t fdt_launch_fit_k3_1(const FitArgs*, hipStream_t);
+#define FDT_LM_DECL(K, P)                                                              \
+  hipError_t fdt_launch_lm_k##K##_##P(const LmArgs*, int kind, int f0, hipStream_t); \
+  hipError_t fdt_launch_tau_k##K##_##P(const TauArgs*, hipStream_t);
+FDT_LM_DECL(1, 0)
+FDT_LM_DECL(2, 0)
+FDT_LM_DECL(3, 0)
+FDT_LM_DECL(1, 1)
+FDT_LM_DECL(2, 1)
+FDT_LM_DECL(3, 1)
+hipError_t fdt_launch_f0_flag(const F0FlagArgs*, hipStream_t);
+hipError_t fdt_launch_zero_ints(int*, int, hipStream_t);
 hipError_t fdt_launch_finish(const FinishArgs*, hipStream_t);
 hipError_t fdt_launch_transpose(const float*, float*, int, int, hipStream_t);
 #define FDT_VPL_DECL(V) hipError_t fdt_launch_mcmc_vpl##V(const McmcArgs*, int, int, hipStream_t);
@@ -148,7 +153,9 @@ struct fdt_ctx {
   int Npad = 0;
   int nvox = 0;
   DevBuf<float> d_data, d_stage, d_params, d_tau, d_energy;
-  DevBuf<int> d_iters;
+  DevBuf<int> d_iters, d_queue, d_list;
+  DevBuf<float> d_s1;             // finalised stage-1 vectors of the model >= 2 pipeline
+  DevBuf<unsigned char> d_refit;  // voxels refitted without f0
   bool have_params = false, have_samples = false, have_finish = false;
   // raw and sorted samples
   DevBuf<float> s_th, s_ph, s_f, s_S0, s_d, s_dstd, s_R, s_f0, s_tau;
@@ -229,6 +236,10 @@ void fdt_destroy(fdt_ctx* c) {
   c->d_pinv.release();
   c->d_acq.release();
   c->d_iters.release();
+  c->d_queue.release();
+  c->d_list.release();
+  c->d_s1.release();
+  c->d_refit.release();
   for (auto& e : c->ev) if (e) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
@@ -292,50 +303,143 @@ int fdt_load_data(fdt_ctx* c, const float* data, int layout, int nvox) {
   return FDT_OK;
 }
 
-static hipError_t launch_fit(const FitArgs& a, int K, int fp64, hipStream_t s) {
+static hipError_t launch_lm(const LmArgs& a, int K, int fp64, int kind, int f0, hipStream_t s) {
   if (!fp64) {
-    if (K == 1) return fdt_launch_fit_k1_0(&a, s);
-    if (K == 2) return fdt_launch_fit_k2_0(&a, s);
-    if (K == 3) return fdt_launch_fit_k3_0(&a, s);
+    if (K == 1) return fdt_launch_lm_k1_0(&a, kind, f0, s);
+    if (K == 2) return fdt_launch_lm_k2_0(&a, kind, f0, s);
+    if (K == 3) return fdt_launch_lm_k3_0(&a, kind, f0, s);
   } else {
-    if (K == 1) return fdt_launch_fit_k1_1(&a, s);
-    if (K == 2) return fdt_launch_fit_k2_1(&a, s);
-    if (K == 3) return fdt_launch_fit_k3_1(&a, s);
+    if (K == 1) return fdt_launch_lm_k1_1(&a, kind, f0, s);
+    if (K == 2) return fdt_launch_lm_k2_1(&a, kind, f0, s);
+    if (K == 3) return fdt_launch_lm_k3_1(&a, kind, f0, s);
   }
   return hipErrorInvalidValue;
 }
 
-static FitArgs base_fit_args(fdt_ctx* c) {
-  FitArgs a{};
+static hipError_t launch_tau(const TauArgs& a, int K, int fp64, hipStream_t s) {
+  if (!fp64) {
+    if (K == 1) return fdt_launch_tau_k1_0(&a, s);
+    if (K == 2) return fdt_launch_tau_k2_0(&a, s);
+    if (K == 3) return fdt_launch_tau_k3_0(&a, s);
+  } else {
+    if (K == 1) return fdt_launch_tau_k1_1(&a, s);
+    if (K == 2) return fdt_launch_tau_k2_1(&a, s);
+    if (K == 3) return fdt_launch_tau_k3_1(&a, s);
+  }
+  return hipErrorInvalidValue;
+}
+
+static LmArgs base_lm_args(fdt_ctx* c) {
+  LmArgs a{};
   a.data = c->d_data.p;
   a.N = c->N;
   a.nvox = c->nvox;
-  a.bvals = c->d_bvals.p;
-  a.bvecs = c->d_bvecs.p;
+  a.acq = c->d_acq.p;
   a.pinvA = c->d_pinv.p;
-  a.model = c->o.model;
-  a.nonlin = c->o.nonlin;
-  a.f0 = c->o.f0;
-  a.rician = c->o.rician;
-  a.Rmean = c->o.R_prior_mean;
+  a.lm = 1;
+  a.R = c->o.R_prior_mean;
   return a;
 }
 
+// The fit() pipeline of CUDA/xfibres_gpu.cu:173-348 as a chain of launches
+// on the context stream: stage 1 (PVM_single, or PVM_single_c for model 1
+// without --nonlinear), PVM_multi for model >= 2, the --f0 refit of
+// implausible voxels over a device list (model 1: the same stage without f0;
+// model >= 2: PVM_single_c then PVM_multi, :331-333), and tau for --rician.
 static int fit_async(fdt_ctx* c) {
   if (c->N <= 0) return set_err(c, FDT_ESTATE, "no acquisition");
-  int np = np_of(c->o);
+  const fdt_opts& o = c->o;
+  const int K = o.nfibres, fp64 = o.lm_fp64 ? 1 : 0, f0 = o.f0 ? 1 : 0;
+  const int np = np_of(o), np1 = 2 + 3 * K + f0;
   size_t nv = std::max(c->nvox, 1);
   HIPCHK(c, c->d_params.ensure(nv * np));
   HIPCHK(c, c->d_tau.ensure(nv));
   HIPCHK(c, c->d_iters.ensure(nv));
-  FitArgs a = base_fit_args(c);
-  a.mode = FIT_PIPELINE;
-  a.out = c->d_params.p;
-  a.out_np = np;
-  a.tau = c->d_tau.p;
-  a.iters = c->d_iters.p;
+  HIPCHK(c, c->d_queue.ensure(NQUEUE));
+  if (o.model >= 2) HIPCHK(c, c->d_s1.ensure(nv * np1));
+  if (f0) {
+    HIPCHK(c, c->d_list.ensure(nv));
+    HIPCHK(c, c->d_refit.ensure(nv));
+  }
+  int* q = c->d_queue.p;
+  const int k1 = o.model == 1 ? (o.nonlin ? KIND_SINGLE : KIND_SINGLE_C) : KIND_SINGLE;
+  const int G = o.model == 3 ? 2 : 0;
   HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
-  HIPCHK(c, launch_fit(a, c->o.nfibres, c->o.lm_fp64, c->stream));
+  HIPCHK(c, fdt_launch_zero_ints(q, NQUEUE, c->stream));
+  LmArgs a = base_lm_args(c);
+  a.rician = o.rician;
+  a.iters = c->d_iters.p;
+  // stage 1 from the tensor start values
+  a.init = INIT_DTI;
+  a.queue = q + 0;
+  a.out = o.model == 1 ? c->d_params.p : c->d_s1.p;
+  a.out_np = o.model == 1 ? np : np1;
+  a.iters_add = 0;
+  HIPCHK(c, launch_lm(a, K, fp64, k1, f0, c->stream));
+  if (o.model >= 2) {
+    LmArgs b = a;
+    b.init = INIT_STAGE1;
+    b.in = c->d_s1.p;
+    b.in_np = np1;
+    b.G = G;
+    b.queue = q + 1;
+    b.out = c->d_params.p;
+    b.out_np = np;
+    b.iters_add = 1;
+    HIPCHK(c, launch_lm(b, K, fp64, KIND_MULTI, f0, c->stream));
+  }
+  if (f0) {
+    F0FlagArgs fa{};
+    fa.params = c->d_params.p;
+    fa.nvox = c->nvox;
+    fa.np = np;
+    fa.fb = o.model >= 2 ? 3 : 2;
+    fa.K = K;
+    fa.list = c->d_list.p;
+    fa.count = q + 2;
+    fa.refit = c->d_refit.p;
+    HIPCHK(c, fdt_launch_f0_flag(&fa, c->stream));
+    LmArgs r = a;
+    r.list = c->d_list.p;
+    r.count = q + 2;
+    r.queue = q + 3;
+    r.iters_add = 1;
+    if (o.model == 1) {
+      r.out = c->d_params.p;
+      r.out_np = np;
+      r.set_f0_slot = 1;
+      HIPCHK(c, launch_lm(r, K, fp64, k1, 0, c->stream));
+    } else {
+      r.out = c->d_s1.p;  // PVM_single_c without f0: 2 + 3K values
+      r.out_np = 2 + 3 * K;
+      HIPCHK(c, launch_lm(r, K, fp64, KIND_SINGLE_C, 0, c->stream));
+      LmArgs m = r;
+      m.init = INIT_STAGE1;
+      m.in = c->d_s1.p;
+      m.in_np = 2 + 3 * K;
+      m.G = G;
+      m.queue = q + 4;
+      m.out = c->d_params.p;
+      m.out_np = np;
+      m.set_f0_slot = 1;
+      HIPCHK(c, launch_lm(m, K, fp64, KIND_MULTI, 0, c->stream));
+    }
+  }
+  if (o.rician) {
+    TauArgs t{};
+    t.data = c->d_data.p;
+    t.N = c->N;
+    t.nvox = c->nvox;
+    t.acq = c->d_acq.p;
+    t.params = c->d_params.p;
+    t.np = np;
+    t.model = o.model;
+    t.f0 = f0;
+    t.R = o.R_prior_mean;
+    t.refit = f0 ? c->d_refit.p : nullptr;
+    t.tau = c->d_tau.p;
+    HIPCHK(c, launch_tau(t, K, fp64, c->stream));
+  }
   HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
   c->timed_fit = true;
   c->have_params = true;
@@ -365,17 +469,19 @@ int fdt_lm_stage(fdt_ctx* c, int kind, int K, int f0, int G, float R, const floa
   HIPCHK(c, dout.ensure(n));
   HIPCHK(c, dit.ensure(c->nvox));
   HIPCHK(c, hipMemcpyAsync(ds.p, start, n * sizeof(float), hipMemcpyHostToDevice, c->stream));
-  FitArgs a = base_fit_args(c);
-  a.mode = FIT_STAGE;
-  a.kind = kind;
+  HIPCHK(c, c->d_queue.ensure(NQUEUE));
+  HIPCHK(c, fdt_launch_zero_ints(c->d_queue.p, NQUEUE, c->stream));
+  LmArgs a = base_lm_args(c);
+  a.init = INIT_BUF;
+  a.in = ds.p;
+  a.in_np = np;
   a.G = G;
   a.R = R;
-  a.stage_f0 = f0;
-  a.start = ds.p;
+  a.queue = c->d_queue.p;
   a.out = dout.p;
   a.out_np = np;
   a.iters = dit.p;
-  HIPCHK(c, launch_fit(a, K, c->o.lm_fp64, c->stream));
+  HIPCHK(c, launch_lm(a, K, c->o.lm_fp64 ? 1 : 0, kind, f0 ? 1 : 0, c->stream));
   HIPCHK(c, hipMemcpyAsync(out, dout.p, n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
   if (iters) HIPCHK(c, hipMemcpyAsync(iters, dit.p, c->nvox * sizeof(int), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -392,13 +498,15 @@ int fdt_lm_start(fdt_ctx* c, int kind, int K, int f0, float* out) {
   size_t n = (size_t)c->nvox * np;
   DevBuf<float> dout;
   HIPCHK(c, dout.ensure(n));
-  FitArgs a = base_fit_args(c);
-  a.mode = FIT_START;
-  a.kind = kind;
-  a.stage_f0 = f0;
+  HIPCHK(c, c->d_queue.ensure(NQUEUE));
+  HIPCHK(c, fdt_launch_zero_ints(c->d_queue.p, NQUEUE, c->stream));
+  LmArgs a = base_lm_args(c);
+  a.init = INIT_DTI;
+  a.lm = 0;
+  a.queue = c->d_queue.p;
   a.out = dout.p;
   a.out_np = np;
-  HIPCHK(c, launch_fit(a, K, c->o.lm_fp64, c->stream));
+  HIPCHK(c, launch_lm(a, K, c->o.lm_fp64 ? 1 : 0, kind, f0 ? 1 : 0, c->stream));
   HIPCHK(c, hipMemcpyAsync(out, dout.p, n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   dout.release();

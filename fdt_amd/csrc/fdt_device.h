@@ -87,6 +87,39 @@ FDT_DEV double wave_sum_d(double v) {
   v += dppd<0x140>(v);
   return (rld(v, 0) + rld(v, 16)) + (rld(v, 32) + rld(v, 48));
 }
+// 16-lane row versions for doubles (every lane of the row ends with the
+// same value: each step combines a lane with its partner symmetrically)
+FDT_DEV double row_sum16_d(double v) {
+  v += dppd<0xB1>(v);
+  v += dppd<0x4E>(v);
+  v += dppd<0x141>(v);
+  v += dppd<0x140>(v);
+  return v;
+}
+FDT_DEV double row_max16_d(double v) {
+  v = fmax(v, dppd<0xB1>(v));
+  v = fmax(v, dppd<0x4E>(v));
+  v = fmax(v, dppd<0x141>(v));
+  v = fmax(v, dppd<0x140>(v));
+  return v;
+}
+template <int L>
+FDT_DEV double row_bcast_d(double v) {
+  static_assert(L >= 0 && L < 16, "row lane");
+  return dppd<0x150 + L>(v);
+}
+template <int L>
+FDT_DEV int row_bcast_i(int v) {
+  static_assert(L >= 0 && L < 16, "row lane");
+  return __builtin_amdgcn_update_dpp(0, v, 0x150 + L, 0xF, 0xF, false);
+}
+// value of v in lane `src` of the wave (ds_bpermute; src may differ per lane)
+FDT_DEV double bperm_d(int src, double v) {
+  int2 b = __builtin_bit_cast(int2, v);
+  b.x = __builtin_amdgcn_ds_bpermute(src * 4, b.x);
+  b.y = __builtin_amdgcn_ds_bpermute(src * 4, b.y);
+  return __builtin_bit_cast(double, b);
+}
 // make a per-lane-identical value provably wave-uniform
 FDT_DEV float uni(float v) { return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, v))); }
 FDT_DEV double uni(double v) {

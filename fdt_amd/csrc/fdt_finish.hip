@@ -140,7 +140,38 @@ __global__ __launch_bounds__(256) void transpose_kernel(const float* __restrict_
   }
 }
 
+// Implausible-f0 test of CUDA/xfibres_gpu.cu:216-246, 305-339 on the fitted
+// parameters: (K > 0 && f_1 < 0.05) || d > 0.007 || f0 > 0.4 puts the voxel on
+// the refit list (list order is irrelevant: voxels are fitted independently).
+__global__ void f0_flag_kernel(F0FlagArgs a) {
+  int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= a.nvox) return;
+  const float* q = a.params + (size_t)v * a.np;
+  float md = q[1], mf = q[a.fb], f0v = q[a.np - 1];
+  bool bad = (a.K > 0 && mf < 0.05f) || md > 0.007f || f0v > 0.4f;
+  a.refit[v] = bad ? 1 : 0;
+  if (bad) a.list[atomicAdd(a.count, 1)] = v;
+}
+
+// work-queue counters to zero with a kernel (capturable in a graph)
+__global__ void zero_ints_kernel(int* p, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
 }  // namespace fdt
+
+extern "C" hipError_t fdt_launch_f0_flag(const fdt::F0FlagArgs* a, hipStream_t s) {
+  if (a->nvox <= 0) return hipSuccess;
+  hipLaunchKernelGGL(fdt::f0_flag_kernel, dim3((a->nvox + 255) / 256), dim3(256), 0, s, *a);
+  return hipGetLastError();
+}
+
+extern "C" hipError_t fdt_launch_zero_ints(int* p, int n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(fdt::zero_ints_kernel, dim3((n + 255) / 256), dim3(256), 0, s, p, n);
+  return hipGetLastError();
+}
 
 extern "C" hipError_t fdt_launch_finish(const fdt::FinishArgs* a, hipStream_t s) {
   if (a->nvox <= 0) return hipSuccess;

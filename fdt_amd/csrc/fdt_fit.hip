@@ -1,49 +1,43 @@
 // fdt_fit.hip -- per-voxel Levenberg-Marquardt initialisation on gfx950.
 //
-// One wave64 per voxel, lanes striding over the diffusion volumes.  The whole
-// per-voxel pipeline of the reference's fit() (CUDA/xfibres_gpu.cu:173-348)
-// runs inside the wave without host round trips:
+// One voxel per 16-lane DPP row, four voxels per wave64, 16 per 256-thread
+// workgroup.  Launches are persistent: each row takes a voxel from a device
+// work queue, runs one LM stage on it to convergence and takes the next, so a
+// slow voxel never holds back the rest of its wave.  The stages of the
+// reference's fit() (CUDA/xfibres_gpu.cu:173-348) are separate launches:
 //   tensor fit (DTI::linfit, diffmodels.cc:17-46, done per voxel on the host
 //   by the reference, CUDA/diffmodels.cu:57-94) -> start values
 //   (CUDA/diffmodels.cu:72-93, 149-183 with PVM_single_c::fit_pvf,
-//   diffmodels.cc:561-603) -> LM stage(s) -> finalisation -> the --f0
-//   implausibility refit -> Rician tau (CUDA/diffmodels.cu:259-347).
-// The residual, Jacobian, gradient (2 J^T r) and Gauss-Newton Hessian
-// (2 J^T J) are accumulated per lane and summed across the wave with DPP row
-// reductions and lane reads; the damped normal equations are solved in FP64 by LU with
-// partial pivoting (the algorithm of CUDA/solver_mult_inverse.cu:14-166) in a
-// per-wave LDS workspace.  The model and Jacobian run in T = float
-// (production) or T = double (parity mode).
+//   diffmodels.cc:561-603) -> PVM_single / PVM_single_c LM -> finalisation;
+//   PVM_multi LM from the stage-1 result; the --f0 implausibility refit over a
+//   device list; Rician tau (CUDA/diffmodels.cu:259-347).
+// Lane l of a row holds volumes l, l+16, ...; the voxel's signal row is
+// staged in LDS next to the workgroup's acquisition table.  The residual,
+// gradient (2 J^T r) and Gauss-Newton Hessian (2 J^T J) are accumulated per
+// lane and summed over the row with DPP; the damped normal equations are
+// solved in FP64 by LU with partial pivoting (the algorithm of
+// CUDA/solver_mult_inverse.cu:14-166) with row i of the matrix in lane i.  The
+// model and Jacobian run in T = float (production) or T = double (parity
+// mode).
 #include "fdt_device.h"
 #include "fdt_kernels.h"
 #include <type_traits>
 
 namespace fdt {
 
-FDT_DEV int lane_id() { return __lane_id(); }
-
-// The voxel's volumes, streamed: the acquisition (gx, gy, gz, b) is staged in
-// LDS for the workgroup, the signal row is read from the voxel-major data.
-// With --rician, non-positive samples are replaced on the fly by the smallest
-// positive one (remove_NonPositive_entries, CUDA/xfibres_gpu.cu:350-367).
-struct Vols {
-  const float* y;
-  const float4* acq;
+// The voxel as seen by its row: lane rl, the voxel's volumes in LDS.
+struct Row {
+  int rl;
   int N;
-  bool san;
-  float ysub;
-  FDT_DEV float yv(int i) const {
-    float v = y[i];
-    return (san && v <= 0.0f) ? ysub : v;
-  }
+  const float4* acq;  // LDS, [N]
+  const float* y;     // LDS, the voxel's signal (sanitised for --rician)
 };
 
 template <class T>
-FDT_DEV T wsum(T v);
-template <>
-FDT_DEV float wsum<float>(float v) { return uni(wave_sum(v)); }
-template <>
-FDT_DEV double wsum<double>(double v) { return uni(wave_sum_d(v)); }
+FDT_DEV T rsum(T v) {
+  if constexpr (std::is_same<T, double>::value) return row_sum16_d(v);
+  else return row_sum16(v);
+}
 
 // ------------------------------------------------------------------ models
 // Forward model, Jacobian row and cost of the three LM models with the
@@ -283,176 +277,186 @@ struct Mdl {
 };
 
 // cost = sum_i (pred_i - y_i)^2 with FP64 accumulation
-template <class T, int KIND, int K, bool F0>
-FDT_DEV double lm_cost(const Vols& V, const T* p, int G, T Rv, T invR) {
-  Mdl<T, KIND, K, F0> m;
-  m.G = G; m.Rv = Rv; m.invR = invR;
-  m.prep(p);
+template <class T, class M>
+FDT_DEV double row_cost(const Row& R, const M& m, const T* p) {
   double c = 0.0;
-  for (int i = lane_id(); i < V.N; i += 64) {
-    float4 q = V.acq[i];
-    T e = m.pred(p, (T)q.w, (T)q.x, (T)q.y, (T)q.z) - (T)V.yv(i);
+  for (int i = R.rl; i < R.N; i += 16) {
+    float4 q = R.acq[i];
+    T e = m.pred(p, (T)q.w, (T)q.x, (T)q.y, (T)q.z) - (T)R.y[i];
     c += (double)e * (double)e;
   }
-  return uni(wave_sum_d(c));
+  return row_sum16_d(c);
 }
 
-// LU with partial pivoting and substitution, distributed over the wave: lane
-// i (< NP) holds row i of the matrix and entry i of the right-hand side.  The
-// elimination order, the pivot rule (first maximum of the updated column
-// below the diagonal; a zero pivot skips elimination) and the interchange of
-// the right-hand side follow CUDA/solver_mult_inverse.cu:14-166.  Pivot
-// search, row swaps and the pivot-row broadcast use v_readlane, so the solve
-// stays in registers.
+// LU with partial pivoting and substitution, row i of the matrix and entry i
+// of the right-hand side in lane i of the row.  The elimination order, the
+// pivot rule (first maximum of the column at and below the diagonal; after a
+// zero pivot no search and no interchange) and the interchange of the
+// right-hand side follow CUDA/solver_mult_inverse.cu:14-166.  The pivot is
+// found with a DPP row maximum and a ballot, rows are interchanged with
+// ds_bpermute and the pivot row is broadcast with DPP row_newbcast.
 template <int NP>
-FDT_DEV void lu_rows(const double (&A)[NP], double b, double (&x)[NP]) {
-  const int lane = lane_id();
+FDT_DEV void lu_row(const double (&A)[NP], double b, double (&x)[NP], int rl) {
+  const int base = __lane_id() & ~15;
   double r[NP];
 #pragma unroll
   for (int l = 0; l < NP; ++l) r[l] = A[l];
   double rb = b;
-  int mu = 0;
-  double big = fabs(rld(r[0], 0));
-#pragma unroll
-  for (int i = 1; i < NP; ++i) {
-    double t = fabs(rld(r[0], i));
-    if (big < t) { big = t; mu = i; }
-  }
-#pragma unroll
-  for (int k = 0; k < NP; ++k) {
-    if (mu != k) {
-#pragma unroll
-      for (int l = 0; l < NP; ++l) {
-        double vk = rld(r[l], k), vm = rld(r[l], mu);
-        r[l] = lane == k ? vm : (lane == mu ? vk : r[l]);
+  bool search = true;
+  [&]<int... ks>(std::integer_sequence<int, ks...>) {
+    ([&] {
+      constexpr int k = ks;
+      int mu = k;
+      if (search) {
+        double v = (rl >= k && rl < NP) ? fabs(r[k]) : -1.0;
+        double m = row_max16_d(v);
+        uint64_t bal = __builtin_amdgcn_ballot_w64(v == m);
+        mu = __builtin_ctz((uint32_t)(bal >> base) & 0xFFFFu);
       }
-      double bk = rld(rb, k), bm = rld(rb, mu);
-      rb = lane == k ? bm : (lane == mu ? bk : rb);
-    }
-    double pr[NP];
+      if (mu != k) {
+        int src = base + (rl == k ? mu : (rl == mu ? k : rl));
 #pragma unroll
-    for (int l = k; l < NP; ++l) pr[l] = rld(r[l], k);
-    const double diag = pr[k];
-    mu = k + 1;
-    big = 0.0;
-    if (diag != 0.0) {
-      if (lane > k && lane < NP) {
+        for (int l = 0; l < NP; ++l) r[l] = bperm_d(src, r[l]);
+        rb = bperm_d(src, rb);
+      }
+      double pr[NP];
+#pragma unroll
+      for (int l = k; l < NP; ++l) pr[l] = row_bcast_d<k>(r[l]);
+      const double diag = pr[k];
+      search = diag != 0.0;
+      if (diag != 0.0 && rl > k && rl < NP) {
         double mult = r[k] / diag;
         r[k] = mult;
 #pragma unroll
         for (int l = k + 1; l < NP; ++l) r[l] = r[l] - mult * pr[l];
       }
-      const int kn = k + 1 < NP ? k + 1 : k;
-#pragma unroll
-      for (int i = k + 1; i < NP; ++i) {
-        double t = fabs(rld(r[kn], i));
-        if (big < t) { big = t; mu = i; }
-      }
-    }
-  }
+    }(), ...);
+  }(std::make_integer_sequence<int, NP>{});
   // forward substitution with the unit lower factor (lane i: L[i][j] = r[j], j < i)
   double acc = rb;
-#pragma unroll
-  for (int j = 0; j < NP; ++j) {
-    double yj = rld(acc, j);
-    if (lane > j && lane < NP) acc = acc - r[j] * yj;
-  }
+  [&]<int... js>(std::integer_sequence<int, js...>) {
+    ([&] {
+      constexpr int j = js;
+      double yj = row_bcast_d<j>(acc);
+      if (rl > j && rl < NP) acc = acc - r[j] * yj;
+    }(), ...);
+  }(std::make_integer_sequence<int, NP>{});
   // back substitution with the upper factor
-#pragma unroll
-  for (int i = NP - 1; i >= 0; --i) {
-    double xi = rld(acc, i) / rld(r[i], i);
-    x[i] = xi;
-    if (lane < i) acc = acc - r[i] * xi;
-  }
+  [&]<int... is>(std::integer_sequence<int, is...>) {
+    ([&] {
+      constexpr int i = NP - 1 - is;
+      double xi = row_bcast_d<i>(acc) / row_bcast_d<i>(r[i]);
+      x[i] = xi;
+      if (rl < i) acc = acc - r[i] * xi;
+    }(), ...);
+  }(std::make_integer_sequence<int, NP>{});
 }
 
-// The Levenberg loop of CUDA/levenberg_marquardt.cu:24-115: lambda0 = 0.1,
-// /10 after a success, x10 after a failure (the Hessian diagonal carries
-// lambda - lambda_old), relative cost test 1e-8, ltol 1e20, 200 successes.
+// Damped Gauss-Newton system of the row's voxel at parameters p: lane a < NP
+// receives row a of 2 J^T J (Hessian Jacobian) and entry a of 2 J^T r.
 template <class T, int KIND, int K, bool F0>
-FDT_DEV int lm_stage(const Vols& V, T* p, int G, T Rv, T invR) {
-  using M = Mdl<T, KIND, K, F0>;
-  constexpr int NP = M::NP;
+FDT_DEV void row_normal_eq(const Row& R, const Mdl<T, KIND, K, F0>& m, const T* p,
+                           double (&Hrow)[Mdl<T, KIND, K, F0>::NP], double& grow) {
+  constexpr int NP = Mdl<T, KIND, K, F0>::NP;
   constexpr int NH = NP * (NP + 1) / 2;
-  const int lane = lane_id();
-  double lambda = 0.1, olambda = 0.0;
-  const double cftol = 1.0e-8, ltol = 1.0e20;
-  bool success = true, end = false;
-  int niter = 0, total = 0;
-  double pcf = lm_cost<T, KIND, K, F0>(V, p, G, Rv, invR), ncf;
-  T step[NP];
-  double Hrow[NP], grow = 0.0;  // lane i < NP: row i of the damped Hessian, gradient entry i
+  T gacc[NP], hacc[NH];
 #pragma unroll
-  for (int i = 0; i < NP; ++i) Hrow[i] = 0.0;
-  while (!(success && niter++ >= 200)) {
+  for (int j = 0; j < NP; ++j) gacc[j] = T(0);
+#pragma unroll
+  for (int j = 0; j < NH; ++j) hacc[j] = T(0);
+  for (int i = R.rl; i < R.N; i += 16) {
+    float4 q = R.acq[i];
+    T Jg[NP], J0h;
+    T sig = m.jrow(p, (T)q.w, (T)q.x, (T)q.y, (T)q.z, Jg, J0h);
+    T diff = sig - (T)R.y[i];
+#pragma unroll
+    for (int a = 0; a < NP; ++a) gacc[a] += T(2) * Jg[a] * diff;
+    int h = 0;
+#pragma unroll
+    for (int a = 0; a < NP; ++a) {
+      T ja = a == 0 ? J0h : Jg[a];
+#pragma unroll
+      for (int c = a; c < NP; ++c) {
+        T jc = c == 0 ? J0h : Jg[c];
+        hacc[h++] += T(2) * (ja * jc);
+      }
+    }
+  }
+  T hsel[NP], gsel = T(0);
+#pragma unroll
+  for (int a = 0; a < NP; ++a) {
+    T ga = rsum<T>(gacc[a]);
+    gsel = R.rl == a ? ga : gsel;
+  }
+  int h = 0;
+#pragma unroll
+  for (int a = 0; a < NP; ++a)
+#pragma unroll
+    for (int c = a; c < NP; ++c) {
+      T hv = rsum<T>(hacc[h++]);
+      if (R.rl == a) hsel[c] = hv;
+      if (R.rl == c) hsel[a] = hv;
+    }
+  grow = (double)gsel;
+#pragma unroll
+  for (int c = 0; c < NP; ++c) Hrow[c] = (double)hsel[c];
+}
+
+// The Levenberg loop of CUDA/levenberg_marquardt.cu:24-115 as a resumable
+// per-row state: lambda0 = 0.1, /10 after a success, x10 after a failure (the
+// Hessian diagonal carries lambda - lambda_old), relative cost test 1e-8,
+// ltol 1e20, stop at the 201st success test.
+template <class T, int KIND, int K, bool F0>
+struct LmState {
+  static constexpr int NP = Mdl<T, KIND, K, F0>::NP;
+  T p[NP];
+  double Hrow[NP], grow;
+  double pcf, lambda, olambda;
+  int niter, total;
+  bool success;
+
+  FDT_DEV void begin(const Row& R, const Mdl<T, KIND, K, F0>& m0) {
+    Mdl<T, KIND, K, F0> m = m0;
+    m.prep(p);
+    pcf = row_cost<T>(R, m, p);
+    lambda = 0.1; olambda = 0.0;
+    niter = 0; total = 0;
+    success = true;
+  }
+  // one trip of the loop; returns true when the stage has ended
+  FDT_DEV bool step(const Row& R, const Mdl<T, KIND, K, F0>& m0) {
+    if (success && niter++ >= 200) return true;
     ++total;
+    Mdl<T, KIND, K, F0> m = m0;
     if (success) {
-      M m;
-      m.G = G; m.Rv = Rv; m.invR = invR;
       m.prep(p);
-      T gacc[NP], hacc[NH];
-#pragma unroll
-      for (int j = 0; j < NP; ++j) gacc[j] = T(0);
-#pragma unroll
-      for (int j = 0; j < NH; ++j) hacc[j] = T(0);
-      for (int i = lane; i < V.N; i += 64) {
-        {
-          float4 q = V.acq[i];
-          T Jg[NP], J0h;
-          T sig = m.jrow(p, (T)q.w, (T)q.x, (T)q.y, (T)q.z, Jg, J0h);
-          T diff = sig - (T)V.yv(i);
-#pragma unroll
-          for (int a = 0; a < NP; ++a) gacc[a] += T(2) * Jg[a] * diff;
-          int h = 0;
-#pragma unroll
-          for (int a = 0; a < NP; ++a) {
-            T ja = a == 0 ? J0h : Jg[a];
-#pragma unroll
-            for (int c = a; c < NP; ++c) {
-              T jc = c == 0 ? J0h : Jg[c];
-              hacc[h++] += T(2) * (ja * jc);
-            }
-          }
-        }
-      }
-#pragma unroll
-      for (int a = 0; a < NP; ++a) {
-        double ga = (double)wsum<T>(gacc[a]);
-        grow = lane == a ? ga : grow;
-      }
-      int h = 0;
-#pragma unroll
-      for (int a = 0; a < NP; ++a)
-#pragma unroll
-        for (int c = a; c < NP; ++c) {
-          double hv = (double)wsum<T>(hacc[h++]);
-          if (lane == a) Hrow[c] = hv;
-          if (lane == c) Hrow[a] = hv;
-        }
+      row_normal_eq<T, KIND, K, F0>(R, m, p, Hrow, grow);
     }
 #pragma unroll
-    for (int i = 0; i < NP; ++i) if (lane == i) Hrow[i] += (lambda - olambda);
+    for (int i = 0; i < NP; ++i) if (R.rl == i) Hrow[i] += (lambda - olambda);
     double sol[NP];
-    lu_rows<NP>(Hrow, grow, sol);
+    lu_row<NP>(Hrow, grow, sol, R.rl);
+    T st[NP];
 #pragma unroll
-    for (int i = 0; i < NP; ++i) step[i] = p[i] - (T)sol[i];
-    ncf = lm_cost<T, KIND, K, F0>(V, step, G, Rv, invR);
+    for (int i = 0; i < NP; ++i) st[i] = p[i] - (T)sol[i];
+    m.prep(st);
+    double ncf = row_cost<T>(R, m, st);
+    bool end = false;
     if ((success = (ncf < pcf))) {
       olambda = 0.0;
 #pragma unroll
-      for (int i = 0; i < NP; ++i) p[i] = step[i];
+      for (int i = 0; i < NP; ++i) p[i] = st[i];
       lambda = lambda / 10.0;
-      if (2.0 * fabs(pcf - ncf) <= cftol * (fabs(pcf) + fabs(ncf) + 2.0e-16)) end = true;
+      if (2.0 * fabs(pcf - ncf) <= 1.0e-8 * (fabs(pcf) + fabs(ncf) + 2.0e-16)) end = true;
       pcf = ncf;
     } else {
       olambda = lambda;
       lambda = lambda * 10.0;
-      if (lambda > ltol) end = true;
+      if (lambda > 1.0e20) end = true;
     }
-    if (end) break;
+    return end;
   }
-  return total;
-}
+};
 
 // ------------------------------------------------------- finalisation
 template <class T, int K>
@@ -524,7 +528,6 @@ FDT_DEV void finalise(T* p) {
     }
   }
 }
-
 // ------------------------------------------------------------ tensor fit
 FDT_DEV void jacobi3(double A[9], double w[3], double V[9]) {
   for (int i = 0; i < 9; ++i) V[i] = (i % 4 == 0) ? 1.0 : 0.0;
@@ -568,19 +571,40 @@ FDT_DEV void jacobi3(double A[9], double w[3], double V[9]) {
 // positive (the reference's newmat sign is unspecified; LM is invariant).
 FDT_DEV void eig3_desc(const double T_[9], double l[3], double v[3][3]) {
   double A[9], w[3], V[9];
+#pragma unroll
   for (int i = 0; i < 9; ++i) A[i] = T_[i];
   jacobi3(A, w, V);
-  int o[3] = {0, 1, 2};
-  // sort descending (stable on ties by index)
-  for (int i = 0; i < 2; ++i)
-    for (int j = 0; j < 2 - i; ++j)
-      if (w[o[j]] < w[o[j + 1]]) { int t = o[j]; o[j] = o[j + 1]; o[j + 1] = t; }
+#pragma unroll
   for (int k = 0; k < 3; ++k) {
-    l[k] = w[o[k]];
-    for (int i = 0; i < 3; ++i) v[k][i] = V[i * 3 + o[k]];
-    int m = 0;
-    for (int i = 1; i < 3; ++i) if (fabs(v[k][i]) > fabs(v[k][m])) m = i;
-    if (v[k][m] < 0) for (int i = 0; i < 3; ++i) v[k][i] = -v[k][i];
+    l[k] = w[k];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) v[k][i] = V[i * 3 + k];
+  }
+  // sort descending, stable on ties (compare-exchange, no register indexing)
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2 - i; ++j) {
+      bool sw = l[j] < l[j + 1];
+      double t = l[j];
+      l[j] = sw ? l[j + 1] : t;
+      l[j + 1] = sw ? t : l[j + 1];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        double u = v[j][c];
+        v[j][c] = sw ? v[j + 1][c] : u;
+        v[j + 1][c] = sw ? u : v[j + 1][c];
+      }
+    }
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    double big = v[k][0];
+#pragma unroll
+    for (int i = 1; i < 3; ++i) big = fabs(v[k][i]) > fabs(big) ? v[k][i] : big;
+    if (big < 0) {
+#pragma unroll
+      for (int i = 0; i < 3; ++i) v[k][i] = -v[k][i];
+    }
   }
 }
 
@@ -594,36 +618,37 @@ FDT_DEV void cart2sph(const double v[3], double& th, double& ph) {
 }
 
 // DTI::linfit (diffmodels.cc:17-46) + calc_tensor_parameters (:239-265)
-FDT_DEV Dti dti_fit(const Vols& V, const double* pinvA) {
-  const int lane = lane_id();
-  const int N = V.N;
+FDT_DEV Dti dti_row(const Row& R, const double* pinvA) {
+  const int N = R.N;
   double ysum = 0;
   float ymx = 0.0f;
-  for (int i = lane; i < N; i += 64) { float y = V.yv(i); ysum += (double)y; ymx = fmaxf(ymx, fabsf(y)); }
-  ysum = uni(wave_sum_d(ysum));
-  double ymax = (double)uni(-wave_min(-ymx));
+  for (int i = R.rl; i < N; i += 16) { float y = R.y[i]; ysum += (double)y; ymx = fmaxf(ymx, fabsf(y)); }
+  ysum = row_sum16_d(ysum);
+  double ymax = (double)row_max16(ymx);
   double mean = ysum / N;
   double D[7];
+#pragma unroll
   for (int j = 0; j < 7; ++j) D[j] = 0;
-  for (int i = lane; i < N; i += 64) {
-    float y = V.yv(i);
+  for (int i = R.rl; i < N; i += 16) {
+    float y = R.y[i];
     double l = y > 0.0f ? log((double)y) : 0.0;
 #pragma unroll
     for (int j = 0; j < 7; ++j) D[j] += pinvA[j * N + i] * l;
   }
 #pragma unroll
-  for (int j = 0; j < 7; ++j) D[j] = -uni(wave_sum_d(D[j]));
+  for (int j = 0; j < 7; ++j) D[j] = -row_sum16_d(D[j]);
   double s0 = (D[6] > -23) ? exp(-D[6]) : ymax;
   if (s0 < mean) s0 = ymax;
+#pragma unroll
   for (int j = 0; j < 7; ++j) D[j] = 0;
-  for (int i = lane; i < N; i += 64) {
-    double y = (double)V.yv(i);
+  for (int i = R.rl; i < N; i += 16) {
+    double y = (double)R.y[i];
     double l = (y / s0) > 0.01 ? log(y) : log(0.01 * s0);
 #pragma unroll
     for (int j = 0; j < 7; ++j) D[j] += pinvA[j * N + i] * l;
   }
 #pragma unroll
-  for (int j = 0; j < 7; ++j) D[j] = -uni(wave_sum_d(D[j]));
+  for (int j = 0; j < 7; ++j) D[j] = -row_sum16_d(D[j]);
   s0 = exp(-D[6]);
   if (s0 < mean) s0 = mean;
   Dti o;
@@ -667,10 +692,10 @@ FDT_DEV void start_single(const Dti& dti, T* p) {
 
 // PVM_single_c start values with fit_pvf (CUDA/diffmodels.cu:149-183,
 // diffmodels.cc:561-603): LLS for the fractions through the pseudo-inverse
-// of the K-column design, formed from wave-reduced M^T M and M^T (y - s0 iso).
+// of the K-column design, formed from row-reduced M^T M and M^T (y - s0 iso).
 template <class T, int K, bool F0>
-FDT_DEV void start_single_c(const Dti& dti, const Vols& V, T* p) {
-  const int N = V.N;
+FDT_DEV void start_single_c(const Dti& dti, const Row& R, T* p) {
+  const int N = R.N;
   constexpr int NP = 2 + 3 * K + (F0 ? 1 : 0);
   double th, ph;
   p[0] = (T)dti.s0;
@@ -693,30 +718,28 @@ FDT_DEV void start_single_c(const Dti& dti, const Vols& V, T* p) {
   for (int a = 0; a < K * K; ++a) MtM[a] = 0;
 #pragma unroll
   for (int a = 0; a < K; ++a) MtY[a] = 0;
-  for (int i = lane_id(); i < N; i += 64) {
-    {
-      float4 q = V.acq[i];
-      T b = (T)q.w;
-      T iso = exp(-b * d);
-      double yi = (double)((T)V.yv(i) - s0 * iso);
-      double m[K];
+  for (int i = R.rl; i < N; i += 16) {
+    float4 q = R.acq[i];
+    T b = (T)q.w;
+    T iso = exp(-b * d);
+    double yi = (double)((T)R.y[i] - s0 * iso);
+    double m[K];
 #pragma unroll
-      for (int k = 0; k < K; ++k) {
-        T dp = (T)q.x * dir[k][0] + (T)q.y * dir[k][1] + (T)q.z * dir[k][2];
-        m[k] = (double)(s0 * (exp(-b * d * dp * dp) - iso));
-      }
+    for (int k = 0; k < K; ++k) {
+      T dp = (T)q.x * dir[k][0] + (T)q.y * dir[k][1] + (T)q.z * dir[k][2];
+      m[k] = (double)(s0 * (exp(-b * d * dp * dp) - iso));
+    }
 #pragma unroll
-      for (int a = 0; a < K; ++a) {
-        MtY[a] += m[a] * yi;
+    for (int a = 0; a < K; ++a) {
+      MtY[a] += m[a] * yi;
 #pragma unroll
-        for (int c = 0; c < K; ++c) MtM[a * K + c] += m[a] * m[c];
-      }
+      for (int c = 0; c < K; ++c) MtM[a * K + c] += m[a] * m[c];
     }
   }
 #pragma unroll
-  for (int a = 0; a < K * K; ++a) MtM[a] = uni(wave_sum_d(MtM[a]));
+  for (int a = 0; a < K * K; ++a) MtM[a] = row_sum16_d(MtM[a]);
 #pragma unroll
-  for (int a = 0; a < K; ++a) MtY[a] = uni(wave_sum_d(MtY[a]));
+  for (int a = 0; a < K; ++a) MtY[a] = row_sum16_d(MtY[a]);
   // pinv(M) y = V diag(1/s^2) V^T M^T y, singular values s <= N*s_max*1e-15 dropped
   double fsd[K];
   {
@@ -725,7 +748,6 @@ FDT_DEV void start_single_c(const Dti& dti, const Vols& V, T* p) {
     for (int a = 0; a < K; ++a)
 #pragma unroll
       for (int c = 0; c < K; ++c) A[a * 3 + c] = MtM[a * K + c];
-    for (int a = K; a < 3; ++a) A[a * 3 + a] = 0.0;
     jacobi3(A, w, Vv);
     double smax = 0;
 #pragma unroll
@@ -801,233 +823,162 @@ FDT_DEV void start_multi(const T* s1, T* p) {
   if constexpr (F0) p[NP - 1] = f2x<T>(s1[NP1 - 1]);
 }
 
-// ---------------------------------------------------------- pipelines
-// model 1: PVM_single (--nonlinear) or PVM_single_c; model >= 2: PVM_single
-// then PVM_multi (refit after an implausible --f0 fit: PVM_single_c then
-// PVM_multi, CUDA/xfibres_gpu.cu:331-333).  Output [S0,d,(dstd),f,th,ph..,(f0)].
-template <class T, int K, bool F0>
-FDT_DEV int run_model1(const Vols& V, const Dti& dti, bool nonlin, T* out) {
-  constexpr int NP = 2 + 3 * K + (F0 ? 1 : 0);
-  T p[NP];
-  int it;
-  if (nonlin) {
-    start_single<T, K, F0>(dti, p);
-    it = lm_stage<T, KIND_SINGLE, K, F0>(V, p, 0, T(0), T(1));
-    finalise<T, KIND_SINGLE, K, F0>(p);
-  } else {
-    start_single_c<T, K, F0>(dti, V, p);
-    it = lm_stage<T, KIND_SINGLE_C, K, F0>(V, p, 0, T(0), T(1));
-    finalise<T, KIND_SINGLE_C, K, F0>(p);
-  }
+// ------------------------------------------------------------ kernels
+// Signal row stride in LDS: N rounded up to whole 16-lane groups, an odd
+// number of them, so that the four rows of a wave fall on different banks.
+FDT_DEV int ystride(int N) { return 16 * (((N + 15) / 16) | 1); }
+
+// lane j < NP of the row stores v[j] (no dynamic register indexing)
+template <int NP, class T>
+FDT_DEV void store_vec(float* dst, const T (&v)[NP], int rl) {
 #pragma unroll
-  for (int i = 0; i < NP; ++i) out[i] = p[i];
-  return it;
+  for (int j = 0; j < NP; ++j)
+    if (rl == j) dst[j] = (float)v[j];
 }
 
-template <class T, int K, bool F0>
-FDT_DEV int run_multi(const Vols& V, const Dti& dti, bool refit, int G, T Rv, T* out) {
-  constexpr int NP1 = 2 + 3 * K + (F0 ? 1 : 0), NP = NP1 + 1;
-  T s1[NP1], p[NP];
-  int it;
-  if (!refit) {
-    start_single<T, K, F0>(dti, s1);
-    it = lm_stage<T, KIND_SINGLE, K, F0>(V, s1, 0, T(0), T(1));
-    finalise<T, KIND_SINGLE, K, F0>(s1);
-  } else {
-    start_single_c<T, K, F0>(dti, V, s1);
-    it = lm_stage<T, KIND_SINGLE_C, K, F0>(V, s1, 0, T(0), T(1));
-    finalise<T, KIND_SINGLE_C, K, F0>(s1);
-  }
-  start_multi<T, K, F0>(s1, p);
-  T invR = T(1) / (T(2) * Rv + T(1));
-  it += lm_stage<T, KIND_MULTI, K, F0>(V, p, G, Rv, invR);
-  finalise<T, KIND_MULTI, K, F0>(p);
+template <class T, int KIND, int K, bool F0>
+__global__ __launch_bounds__(256) void lm_kernel(LmArgs a) {
+  extern __shared__ float4 lds4[];
+  float4* sacq = lds4;
+  float* sy = reinterpret_cast<float*>(lds4 + a.N);
+  for (int i = threadIdx.x; i < a.N; i += blockDim.x) sacq[i] = a.acq[i];
+  __syncthreads();
+  using M = Mdl<T, KIND, K, F0>;
+  constexpr int NP = M::NP;
+  const int rl = __lane_id() & 15;
+  float* ym = sy + (threadIdx.x >> 4) * ystride(a.N);
+  Row R;
+  R.rl = rl; R.N = a.N; R.acq = sacq; R.y = ym;
+  M m0;
+  m0.G = a.G;
+  m0.Rv = (T)a.R;
+  m0.invR = T(1) / (T(2) * (T)a.R + T(1));
+  const int count = a.count ? *a.count : a.nvox;
+  LmState<T, KIND, K, F0> S;
+  int vox = 0;
+  bool fresh = true;
+  for (;;) {
+    if (fresh) {
+      int q = 0;
+      if (rl == 0) q = atomicAdd(a.queue, 1);
+      q = row_bcast_i<0>(q);
+      if (q >= count) break;
+      vox = a.list ? a.list[q] : q;
+      // the voxel's signal row, non-positive samples replaced by the smallest
+      // positive one for --rician (remove_NonPositive_entries, CUDA/xfibres_gpu.cu:350-367)
+      const float* src = a.data + (size_t)vox * a.N;
+      float mn = 3.0e38f, mx = -3.0e38f, mnp = 3.0e38f;
+      for (int i = rl; i < a.N; i += 16) {
+        float v = src[i];
+        ym[i] = v;
+        mn = fminf(mn, v); mx = fmaxf(mx, v);
+        if (v > 0.0f) mnp = fminf(mnp, v);
+      }
+      if (a.rician) {
+        mn = row_min16(mn); mx = row_max16(mx); mnp = row_min16(mnp);
+        if (mn <= 0.0f && mx > 0.0f)
+          for (int i = rl; i < a.N; i += 16) if (ym[i] <= 0.0f) ym[i] = mnp;
+      }
+      if (a.init == INIT_BUF) {
 #pragma unroll
-  for (int i = 0; i < NP; ++i) out[i] = p[i];
-  return it;
+        for (int i = 0; i < NP; ++i) S.p[i] = (T)a.in[(size_t)vox * a.in_np + i];
+      } else if constexpr (KIND == KIND_MULTI) {
+        T s1[NP - 1];
+#pragma unroll
+        for (int i = 0; i < NP - 1; ++i) s1[i] = (T)a.in[(size_t)vox * a.in_np + i];
+        start_multi<T, K, F0>(s1, S.p);
+      } else {
+        Dti dti = dti_row(R, a.pinvA);
+        if constexpr (KIND == KIND_SINGLE) start_single<T, K, F0>(dti, S.p);
+        else start_single_c<T, K, F0>(dti, R, S.p);
+      }
+      if (!a.lm) {
+        store_vec<NP>(a.out + (size_t)vox * a.out_np, S.p, rl);
+        continue;
+      }
+      S.begin(R, m0);
+      fresh = false;
+    }
+    if (S.step(R, m0)) {
+      finalise<T, KIND, K, F0>(S.p);
+      store_vec<NP>(a.out + (size_t)vox * a.out_np, S.p, rl);
+      if (a.set_f0_slot && rl == 15) a.out[(size_t)vox * a.out_np + a.out_np - 1] = 0.001f;
+      if (a.iters && rl == 0) a.iters[vox] = (a.iters_add ? a.iters[vox] : 0) + S.total;
+      fresh = true;
+    }
+  }
 }
 
-// 1/var of the residuals of the final fit (N-1 normalisation)
+// 1/var of the residuals of the final fit (N-1 normalisation), one row per voxel
 template <class T, int K>
-FDT_DEV T tau_of(const Vols& V, int model, bool f0, const T* q, T Rv) {
+__global__ __launch_bounds__(256) void tau_kernel(TauArgs a) {
+  __shared__ float4 sacq[MAX_N];
+  for (int i = threadIdx.x; i < a.N; i += blockDim.x) sacq[i] = a.acq[i];
+  __syncthreads();
+  const int rl = __lane_id() & 15;
+  const int vox = blockIdx.x * 16 + (threadIdx.x >> 4);
+  if (vox >= a.nvox) return;
+  const float* y = a.data + (size_t)vox * a.N;
+  float mn = 3.0e38f, mx = -3.0e38f, mnp = 3.0e38f;
+  for (int i = rl; i < a.N; i += 16) {
+    float v = y[i];
+    mn = fminf(mn, v); mx = fmaxf(mx, v);
+    if (v > 0.0f) mnp = fminf(mnp, v);
+  }
+  mn = row_min16(mn); mx = row_max16(mx); mnp = row_min16(mnp);
+  const bool san = (mn <= 0.0f && mx > 0.0f);
+  const float* q = a.params + (size_t)vox * a.np;
+  const int model = a.model;
+  const bool f0 = a.f0 && !(a.refit && a.refit[vox]);
   const int fb = model >= 2 ? 3 : 2;
-  const int N = V.N;
-  int npo = fb + 3 * K + (f0 ? 1 : 0);
-  T S0 = model >= 2 ? fabs(q[0]) : q[0];
-  T dd = q[1], a_ = T(0), b_ = T(0);
-  if (model >= 2) { a_ = q[1] * q[1] / q[2] / q[2]; b_ = q[2] * q[2] / q[1]; }
+  const T Rv = (T)a.R;
+  T S0 = model >= 2 ? fabs((T)q[0]) : (T)q[0];
+  T dd = (T)q[1], a_ = T(0), b_ = T(0);
+  if (model >= 2) { a_ = (T)q[1] * (T)q[1] / (T)q[2] / (T)q[2]; b_ = (T)q[2] * (T)q[2] / (T)q[1]; }
   T iR = T(1) / (T(2) * Rv + T(1));
-  T f0v = f0 ? q[npo - 1] : T(0);
-  T sumf = T(0), x[K][3];
+  T f0v = f0 ? (T)q[a.np - 1] : T(0);
+  T sumf = T(0), x[K][3], fk[K];
 #pragma unroll
   for (int k = 0; k < K; ++k) {
-    sumf += q[fb + 3 * k];
-    T th = q[fb + 3 * k + 1], ph = q[fb + 3 * k + 2];
+    fk[k] = (T)q[fb + 3 * k];
+    sumf += fk[k];
+    T th = (T)q[fb + 3 * k + 1], ph = (T)q[fb + 3 * k + 2];
     x[k][0] = sin(th) * cos(ph); x[k][1] = sin(th) * sin(ph); x[k][2] = cos(th);
   }
   // two passes: mean, then centred sum of squares
-  double sums[2] = {0.0, 0.0}, mean = 0.0;
+  double mean = 0.0, ss = 0.0;
   for (int pass = 0; pass < 2; ++pass) {
     double acc = 0.0;
-    for (int i = lane_id(); i < N; i += 64) {
-      float4 qq = V.acq[i];
-      T b = (T)qq.w;
+    for (int i = rl; i < a.N; i += 16) {
+      float4 g = sacq[i];
+      T b = (T)g.w;
       T iso = model >= 2 ? exp(-a_ * log(T(1) + b * b_)) : exp(-b * dd);
       T an = T(0);
 #pragma unroll
       for (int k = 0; k < K; ++k) {
-        T dp = (T)qq.x * x[k][0] + (T)qq.y * x[k][1] + (T)qq.z * x[k][2];
+        T dp = (T)g.x * x[k][0] + (T)g.y * x[k][1] + (T)g.z * x[k][2];
         T e;
         if (model == 1) e = exp(-b * dd * dp * dp);
         else if (model == 3) e = exp(-b * T(3) * a_ * b_ * iR * ((T(1) - Rv) * dp * dp + Rv));
         else e = exp(-a_ * log(T(1) + b * b_ * (dp * dp)));
-        an += q[fb + 3 * k] * e;
+        an += fk[k] * e;
       }
       T pred = S0 * (f0v + (T(1) - sumf - f0v) * iso + an);
-      double r = (double)V.yv(i) - (double)pred;
+      float yv = y[i];
+      if (san && yv <= 0.0f) yv = mnp;
+      double r = (double)yv - (double)pred;
       acc += pass == 0 ? r : (r - mean) * (r - mean);
     }
-    sums[pass] = uni(wave_sum_d(acc));
-    if (pass == 0) mean = sums[0] / N;
+    acc = row_sum16_d(acc);
+    if (pass == 0) mean = acc / a.N; else ss = acc;
   }
-  return (T)(1.0 / (sums[1] / (N - 1)));
-}
-
-template <class T, int K>
-FDT_DEV void fit_pipeline(const FitArgs& a, const Vols& V, int vox) {
-  const int lane = lane_id();
-  Dti dti = dti_fit(V, a.pinvA);
-  T out[NPMAX];
-  int npo = a.out_np;
-  int it = 0;
-  bool used_f0 = a.f0 != 0;
-  T Rv = (T)a.Rmean;
-  if (a.model == 1) {
-    if (a.f0) {
-      it = run_model1<T, K, true>(V, dti, a.nonlin, out);
-      T md = out[1], mf = out[2], f0v = out[npo - 1];
-      if ((K > 0 && mf < T(0.05)) || md > T(0.007) || f0v > T(0.4)) {
-        it += run_model1<T, K, false>(V, dti, a.nonlin, out);
-        out[npo - 1] = T(0.001);
-        used_f0 = false;
-      }
-    } else {
-      it = run_model1<T, K, false>(V, dti, a.nonlin, out);
-    }
-  } else {
-    int G = a.model == 3 ? 2 : 0;
-    if (a.f0) {
-      it = run_multi<T, K, true>(V, dti, false, G, Rv, out);
-      T md = out[1], mf = out[3], f0v = out[npo - 1];
-      if ((K > 0 && mf < T(0.05)) || md > T(0.007) || f0v > T(0.4)) {
-        it += run_multi<T, K, false>(V, dti, true, G, Rv, out);
-        out[npo - 1] = T(0.001);
-        used_f0 = false;
-      }
-    } else {
-      it = run_multi<T, K, false>(V, dti, false, G, Rv, out);
-    }
-  }
-  T tau = T(0);
-  if (a.rician) tau = tau_of<T, K>(V, a.model, used_f0, out, Rv);
-  if (lane < npo) {
-    T v = out[0];
-#pragma unroll
-    for (int i = 1; i < NPMAX; ++i) if (i == lane) v = out[i];
-    a.out[(size_t)vox * npo + lane] = (float)v;
-  }
-  if (lane == 0) {
-    if (a.tau) a.tau[vox] = (float)tau;
-    if (a.iters) a.iters[vox] = it;
-  }
-}
-
-template <class T, int K, int KIND, bool F0>
-FDT_DEV void fit_stage(const FitArgs& a, const Vols& V, int vox) {
-  constexpr int NP = Mdl<T, KIND, K, F0>::NP;
-  const int lane = lane_id();
-  T p[NP];
-#pragma unroll
-  for (int i = 0; i < NP; ++i) p[i] = (T)a.start[(size_t)vox * NP + i];
-  T Rv = (T)a.R;
-  int it = lm_stage<T, KIND, K, F0>(V, p, a.G, Rv, T(1) / (T(2) * Rv + T(1)));
-  finalise<T, KIND, K, F0>(p);
-  if (lane < NP) {
-    T v = p[0];
-#pragma unroll
-    for (int i = 1; i < NP; ++i) if (i == lane) v = p[i];
-    a.out[(size_t)vox * NP + lane] = (float)v;
-  }
-  if (lane == 0 && a.iters) a.iters[vox] = it;
-}
-
-template <class T, int K, bool F0>
-FDT_DEV void fit_start(const FitArgs& a, const Vols& V, int vox) {
-  constexpr int NP = 2 + 3 * K + (F0 ? 1 : 0);
-  const int lane = lane_id();
-  Dti dti = dti_fit(V, a.pinvA);
-  T p[NP];
-  if (a.kind == KIND_SINGLE) start_single<T, K, F0>(dti, p);
-  else start_single_c<T, K, F0>(dti, V, p);
-  if (lane < NP) {
-    T v = p[0];
-#pragma unroll
-    for (int i = 1; i < NP; ++i) if (i == lane) v = p[i];
-    a.out[(size_t)vox * NP + lane] = (float)v;
-  }
-}
-
-template <int K, bool FP64>
-__global__ __launch_bounds__(64 * FIT_WAVES) void fit_kernel(FitArgs a) {
-  using T = typename std::conditional<FP64, double, float>::type;
-  __shared__ float4 sacq[MAX_N];
-  for (int i = threadIdx.x; i < a.N; i += blockDim.x)
-    sacq[i] = make_float4(a.bvecs[i], a.bvecs[a.N + i], a.bvecs[2 * a.N + i], a.bvals[i]);
-  __syncthreads();
-  const int wave = threadIdx.x / 64;
-  const int lane = lane_id();
-  const int vox = blockIdx.x * FIT_WAVES + wave;
-  if (vox >= a.nvox) return;
-  Vols V;
-  V.y = a.data + (size_t)vox * a.N;
-  V.acq = sacq;
-  V.N = a.N;
-  V.san = false;
-  V.ysub = 0.0f;
-  if (a.rician && a.mode == FIT_PIPELINE) {
-    float mn = 3.0e38f, mx = -3.0e38f, mnp = 3.0e38f;
-    for (int i = lane; i < a.N; i += 64) {
-      float v = V.y[i];
-      mn = fminf(mn, v); mx = fmaxf(mx, v);
-      if (v > 0.0f) mnp = fminf(mnp, v);
-    }
-    mn = uni(wave_min(mn)); mx = uni(-wave_min(-mx)); mnp = uni(wave_min(mnp));
-    V.san = (mn <= 0.0f && mx > 0.0f);
-    V.ysub = mnp;
-  }
-  if (a.mode == FIT_PIPELINE) {
-    fit_pipeline<T, K>(a, V, vox);
-  } else if (a.mode == FIT_STAGE) {
-    if (a.kind == KIND_SINGLE) {
-      if (a.stage_f0) fit_stage<T, K, KIND_SINGLE, true>(a, V, vox);
-      else fit_stage<T, K, KIND_SINGLE, false>(a, V, vox);
-    } else if (a.kind == KIND_SINGLE_C) {
-      if (a.stage_f0) fit_stage<T, K, KIND_SINGLE_C, true>(a, V, vox);
-      else fit_stage<T, K, KIND_SINGLE_C, false>(a, V, vox);
-    } else {
-      if (a.stage_f0) fit_stage<T, K, KIND_MULTI, true>(a, V, vox);
-      else fit_stage<T, K, KIND_MULTI, false>(a, V, vox);
-    }
-  } else {
-    if (a.stage_f0) fit_start<T, K, true>(a, V, vox);
-    else fit_start<T, K, false>(a, V, vox);
-  }
+  if (rl == 0) a.tau[vox] = (float)(1.0 / (ss / (a.N - 1)));
 }
 
 }  // namespace fdt
 
-// One instantiation per translation unit (K x precision), compiled in
-// parallel: -DFDT_FIT_K=1..3 -DFDT_FIT_FP64=0/1.
+// One translation unit per (K, precision), compiled in parallel:
+// -DFDT_FIT_K=1..3 -DFDT_FIT_FP64=0/1.
 #ifndef FDT_FIT_K
 #define FDT_FIT_K 3
 #endif
@@ -1036,9 +987,41 @@ __global__ __launch_bounds__(64 * FIT_WAVES) void fit_kernel(FitArgs a) {
 #endif
 #define FDT_CAT3_(a, b, c) a##b##_##c
 #define FDT_CAT3(a, b, c) FDT_CAT3_(a, b, c)
-extern "C" hipError_t FDT_CAT3(fdt_launch_fit_k, FDT_FIT_K, FDT_FIT_FP64)(const fdt::FitArgs* a, hipStream_t s) {
+using FitReal = std::conditional<FDT_FIT_FP64 != 0, double, float>::type;
+
+namespace {
+// persistent grid: as many workgroups as stay resident, at most one row per voxel
+template <int KIND, bool F0>
+hipError_t launch_lm(const fdt::LmArgs* a, hipStream_t s) {
+  auto kern = fdt::lm_kernel<FitReal, KIND, FDT_FIT_K, F0>;
+  size_t lds = (size_t)a->N * sizeof(float4) + (size_t)fdt::LM_ROWS * (16 * (((a->N + 15) / 16) | 1)) * sizeof(float);
+  int dev = 0, cus = 0, per_cu = 0;
+  hipError_t e;
+  if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
+  if ((e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess) return e;
+  if ((e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 256, lds)) != hipSuccess) return e;
+  long want = ((long)a->nvox + fdt::LM_ROWS - 1) / fdt::LM_ROWS;
+  long cap = (long)(per_cu > 0 ? per_cu : 1) * cus;
+  int grid = (int)(want < cap ? want : cap);
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(256), lds, s, *a);
+  return hipGetLastError();
+}
+}  // namespace
+
+extern "C" hipError_t FDT_CAT3(fdt_launch_lm_k, FDT_FIT_K, FDT_FIT_FP64)(const fdt::LmArgs* a, int kind, int f0,
+                                                                       hipStream_t s) {
   if (a->nvox <= 0) return hipSuccess;
-  dim3 grid((a->nvox + fdt::FIT_WAVES - 1) / fdt::FIT_WAVES), block(64 * fdt::FIT_WAVES);
-  hipLaunchKernelGGL((fdt::fit_kernel<FDT_FIT_K, FDT_FIT_FP64 != 0>), grid, block, 0, s, *a);
+  if (a->N > fdt::MAX_N) return hipErrorInvalidValue;
+  if (kind == fdt::KIND_SINGLE) return f0 ? launch_lm<fdt::KIND_SINGLE, true>(a, s) : launch_lm<fdt::KIND_SINGLE, false>(a, s);
+  if (kind == fdt::KIND_SINGLE_C) return f0 ? launch_lm<fdt::KIND_SINGLE_C, true>(a, s) : launch_lm<fdt::KIND_SINGLE_C, false>(a, s);
+  if (kind == fdt::KIND_MULTI) return f0 ? launch_lm<fdt::KIND_MULTI, true>(a, s) : launch_lm<fdt::KIND_MULTI, false>(a, s);
+  return hipErrorInvalidValue;
+}
+
+extern "C" hipError_t FDT_CAT3(fdt_launch_tau_k, FDT_FIT_K, FDT_FIT_FP64)(const fdt::TauArgs* a, hipStream_t s) {
+  if (a->nvox <= 0) return hipSuccess;
+  if (a->N > fdt::MAX_N) return hipErrorInvalidValue;
+  dim3 grid((a->nvox + 15) / 16), block(256);
+  hipLaunchKernelGGL((fdt::tau_kernel<FitReal, FDT_FIT_K>), grid, block, 0, s, *a);
   return hipGetLastError();
 }

@@ -6,29 +6,58 @@
 namespace fdt {
 
 enum { KIND_SINGLE = 0, KIND_SINGLE_C = 1, KIND_MULTI = 2 };
-enum { FIT_PIPELINE = 0, FIT_STAGE = 1, FIT_START = 2 };
+// how an LM launch obtains each voxel's start vector
+enum { INIT_DTI = 0, INIT_BUF = 1, INIT_STAGE1 = 2 };
 
-constexpr int LM_VM = 8;       // volumes per lane in the LM kernel: N <= 512
 constexpr int NPMAX = 14;      // LM parameters for K <= 3
 constexpr int MAX_N = 512;     // acquisition length supported
-constexpr int FIT_WAVES = 4;   // waves (voxels) per LM workgroup
+constexpr int LM_ROWS = 16;    // voxels (16-lane rows) per 256-thread LM workgroup
+constexpr int NQUEUE = 8;      // work-queue counters per context
 
-struct FitArgs {
-  const float* data;   // voxel-major [nvox][N]
+// One LM stage over a voxel work list (fdt_fit.hip).  Rows of 16 lanes take
+// voxels from a device queue until it is empty.
+struct LmArgs {
+  const float* data;    // voxel-major [nvox][N]
   int N, nvox;
-  const float* bvals;  // [N]
-  const float* bvecs;  // [3][N]
-  const double* pinvA; // [7][N] pseudo-inverse of the tensor design matrix
-  int mode;            // FIT_PIPELINE / FIT_STAGE / FIT_START
-  int model, nonlin, f0, rician;
-  float Rmean;
-  int kind, G, stage_f0;  // FIT_STAGE / FIT_START
-  float R;
-  const float* start;  // FIT_STAGE: [nvox][np]
-  float* out;          // [nvox][out_np]
+  const float4* acq;    // [N] (gx, gy, gz, b)
+  const double* pinvA;  // [7][N] pseudo-inverse of the tensor design matrix
+  const int* list;      // voxel indices, or null for 0..nvox-1
+  const int* count;     // number of list entries (device), or null for nvox
+  int* queue;           // work-queue counter, zero at launch
+  int init;             // INIT_DTI (tensor start values) / INIT_BUF / INIT_STAGE1
+  int lm;               // 0: write the start vector only
+  int G;                // PVM_multi variant (0 gamma, 1 gamma for the ball only, 2 zeppelin)
+  float R;              // zeppelin R
+  int rician;           // replace non-positive samples before fitting
+  const float* in;      // INIT_BUF: [vox][in_np] start vectors; INIT_STAGE1: finalised stage-1 vectors
+  int in_np;
+  float* out;           // [vox][out_np]: finalised parameters (or the start vector when lm == 0)
   int out_np;
-  float* tau;          // FIT_PIPELINE with --rician
-  int* iters;          // optional [nvox]
+  int set_f0_slot;      // also write 0.001 into the last slot (refit without f0)
+  int* iters;           // optional [nvox]: LM iterations
+  int iters_add;        // add to iters instead of overwriting
+};
+
+// implausible-f0 test and refit list (CUDA/xfibres_gpu.cu:216-246, 305-339)
+struct F0FlagArgs {
+  const float* params;
+  int nvox, np, fb;
+  int K;
+  int* list;
+  int* count;
+  unsigned char* refit;
+};
+
+// Rician precision from the residuals of the final fit (CUDA/diffmodels.cu:259-347)
+struct TauArgs {
+  const float* data;
+  int N, nvox;
+  const float4* acq;
+  const float* params;
+  int np, model, f0;
+  float R;
+  const unsigned char* refit;  // voxels refitted without f0, or null
+  float* tau;
 };
 
 struct McmcArgs {

@@ -144,6 +144,11 @@ int fdt_lm_stage(fdt_ctx* c, int kind, int K, int f0, int G, float R, const floa
  * 149-183): out [nvox][np] for kind 0 (PVM_single) or 1 (PVM_single_c). */
 int fdt_lm_start(fdt_ctx* c, int kind, int K, int f0, float* out);
 
+/* Read / replace the LM result that starts the chains: params [nvox][np]
+ * and tau [nvox] (may be NULL).  In the reference this hand-over is internal
+ * to xfibres_gpu() (fit() output into init_Fibres_Multifibres_kernel,
+ * CUDA/xfibres_gpu.cu:88-128); it is exposed so a caller can start the chains
+ * from its own initialisation. */
 int fdt_get_params(fdt_ctx* c, float* params, float* tau);
 int fdt_set_params(fdt_ctx* c, const float* params, const float* tau);
 
@@ -153,15 +158,23 @@ int fdt_set_params(fdt_ctx* c, const float* params, const float* tau);
  * voxels are split into calls. */
 int fdt_mcmc(fdt_ctx* c, uint64_t vox_offset);
 
+/* Sort fibres by the sum of their f samples, means, dyads and dispersion on
+ * the device (record_finish_voxels, CUDA/xfibres_gpu.cu:659-738, and
+ * Samples::record/finish_voxel/save, CUDA/samples.cu:20-347). */
 int fdt_finish(fdt_ctx* c);
+/* Copy the finished outputs into caller-owned host buffers (the arrays that
+ * Samples::save writes as <name>J files, CUDA/samples.cu:262-347); NULL
+ * fields are skipped. */
 int fdt_get_outputs(fdt_ctx* c, const fdt_outputs* out);
 
-/* fit + mcmc + finish on the resident data, asynchronous on the context stream. */
+/* fit + mcmc + finish on the resident data, asynchronous on the context
+ * stream: the per-part pipeline of xfibres_gpu(), CUDA/xfibres_gpu.cu:29-146. */
 int fdt_run(fdt_ctx* c, uint64_t vox_offset);
 int fdt_sync(fdt_ctx* c);
 int fdt_get_timing(fdt_ctx* c, fdt_timing* t);
 
-/* Force the MCMC volumes-per-lane tile (0 = automatic). */
+/* Force the MCMC volumes-per-lane tile (0 = automatic).  No counterpart in
+ * the reference (its MCMC block shape is fixed, CUDA/runmcmc.cu:108-316). */
 int fdt_set_mcmc_tile(fdt_ctx* c, int vpl);
 
 #ifdef __cplusplus
