@@ -23,7 +23,10 @@ KIND_SINGLE, KIND_SINGLE_C, KIND_MULTI = 0, 1, 2
 EXPORTS = ["fdt_default_opts", "fdt_create", "fdt_destroy", "fdt_last_error", "fdt_set_acquisition",
            "fdt_load_data", "fdt_num_params", "fdt_fit", "fdt_lm_stage", "fdt_lm_start", "fdt_get_params",
            "fdt_set_params", "fdt_mcmc", "fdt_finish", "fdt_get_outputs", "fdt_run", "fdt_sync",
-           "fdt_get_timing", "fdt_set_mcmc_tile"]
+           "fdt_get_timing", "fdt_set_mcmc_tile",
+           "fdt_queue_create", "fdt_queue_destroy", "fdt_queue_last_error", "fdt_enqueue_batch", "fdt_run_queue",
+           "fdt_queue_num_jobs", "fdt_queue_get_outputs", "fdt_queue_get_params", "fdt_queue_set_tiles",
+           "fdt_queue_autotune", "fdt_queue_timing"]
 
 
 class Opts(C.Structure):
@@ -88,6 +91,20 @@ def lib():
         L.fdt_sync.argtypes = [vp]
         L.fdt_get_timing.argtypes = [vp, C.POINTER(Timing)]
         L.fdt_set_mcmc_tile.argtypes = [vp, C.c_int]
+        L.fdt_queue_create.argtypes = [C.c_int]
+        L.fdt_queue_create.restype = vp
+        L.fdt_queue_destroy.argtypes = [vp]
+        L.fdt_queue_destroy.restype = None
+        L.fdt_queue_last_error.argtypes = [vp]
+        L.fdt_queue_last_error.restype = C.c_char_p
+        L.fdt_enqueue_batch.argtypes = [vp, C.POINTER(Opts), C.c_int, _FP, _FP, _FP, C.c_int, C.c_int, C.c_uint64]
+        L.fdt_run_queue.argtypes = [vp]
+        L.fdt_queue_num_jobs.argtypes = [vp]
+        L.fdt_queue_get_outputs.argtypes = [vp, C.c_int, C.POINTER(Outputs)]
+        L.fdt_queue_get_params.argtypes = [vp, C.c_int, _FP, _FP]
+        L.fdt_queue_set_tiles.argtypes = [vp, C.c_int, C.c_int, C.c_int]
+        L.fdt_queue_autotune.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_float)]
+        L.fdt_queue_timing.argtypes = [vp, C.POINTER(C.c_float), ip, ip, C.c_int]
         _lib = L
     return _lib
 
@@ -211,35 +228,110 @@ class Xfibres:
         self._chk(self.L.fdt_set_mcmc_tile(self.h, vpl))
 
     def outputs(self, raw_scalars=True):
-        K, ns, nv = self.K, self.nsamples, self.nvox
-        o = self.o
-        arr = dict(th=np.zeros((K, ns, nv), np.float32), ph=np.zeros((K, ns, nv), np.float32),
-                   f=np.zeros((K, ns, nv), np.float32), mean_S0=np.zeros(nv, np.float32),
-                   mean_d=np.zeros(nv, np.float32), mean_f=np.zeros((K, nv), np.float32),
-                   dyads=np.zeros((K, 3, nv), np.float32), disp=np.zeros((K, nv), np.float32))
-        if o.model >= 2:
-            arr["mean_dstd"] = np.zeros(nv, np.float32)
-        if o.model == 3:
-            arr["mean_R"] = np.zeros(nv, np.float32)
-        if o.f0:
-            arr["mean_f0"] = np.zeros(nv, np.float32)
-        if o.rician:
-            arr["mean_tau"] = np.zeros(nv, np.float32)
-        if raw_scalars:
-            arr["S0"] = np.zeros((ns, nv), np.float32)
-            arr["d"] = np.zeros((ns, nv), np.float32)
-            if o.model >= 2:
-                arr["dstd"] = np.zeros((ns, nv), np.float32)
-            if o.model == 3:
-                arr["R"] = np.zeros((ns, nv), np.float32)
-            if o.f0:
-                arr["f0"] = np.zeros((ns, nv), np.float32)
-            if o.rician:
-                arr["tau"] = np.zeros((ns, nv), np.float32)
-        out = Outputs()
-        for n in _OUT_FIELDS:
-            setattr(out, n, arr[n].ctypes.data_as(_FP) if n in arr else _FP())
+        arr, out = _alloc_outputs(self.o, self.nvox, raw_scalars)
         self._chk(self.L.fdt_get_outputs(self.h, C.byref(out)))
+        return arr
+
+
+def _alloc_outputs(o, nv, raw_scalars=True):
+    """numpy buffers for every output the options produce, and the fdt_outputs view of them"""
+    K, ns = o.nfibres, o.njumps // o.sampleevery
+    arr = dict(th=np.zeros((K, ns, nv), np.float32), ph=np.zeros((K, ns, nv), np.float32),
+               f=np.zeros((K, ns, nv), np.float32), mean_S0=np.zeros(nv, np.float32),
+               mean_d=np.zeros(nv, np.float32), mean_f=np.zeros((K, nv), np.float32),
+               dyads=np.zeros((K, 3, nv), np.float32), disp=np.zeros((K, nv), np.float32))
+    if o.model >= 2:
+        arr["mean_dstd"] = np.zeros(nv, np.float32)
+    if o.model == 3:
+        arr["mean_R"] = np.zeros(nv, np.float32)
+    if o.f0:
+        arr["mean_f0"] = np.zeros(nv, np.float32)
+    if o.rician:
+        arr["mean_tau"] = np.zeros(nv, np.float32)
+    if raw_scalars:
+        arr["S0"] = np.zeros((ns, nv), np.float32)
+        arr["d"] = np.zeros((ns, nv), np.float32)
+        if o.model >= 2:
+            arr["dstd"] = np.zeros((ns, nv), np.float32)
+        if o.model == 3:
+            arr["R"] = np.zeros((ns, nv), np.float32)
+        if o.f0:
+            arr["f0"] = np.zeros((ns, nv), np.float32)
+        if o.rician:
+            arr["tau"] = np.zeros((ns, nv), np.float32)
+    out = Outputs()
+    for n in _OUT_FIELDS:
+        setattr(out, n, arr[n].ctypes.data_as(_FP) if n in arr else _FP())
+    return arr, out
+
+
+class Queue:
+    """A batch of volumes with their own acquisitions, models and fibre counts
+    run together: concurrent LM stages, one persistent chain launch over all
+    volumes, hipGraph replay from the second run on (fdt_*queue* of the C ABI)."""
+
+    def __init__(self, device: int = 0):
+        self.L = lib()
+        self.h = self.L.fdt_queue_create(device)
+        if not self.h:
+            raise FdtError("fdt_queue_create failed (no HIP device)")
+        self.jobs = []
+
+    def close(self):
+        if self.h:
+            self.L.fdt_queue_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def _chk(self, rc):
+        if rc < 0:
+            raise FdtError(self.L.fdt_queue_last_error(self.h).decode())
+        return rc
+
+    def enqueue(self, bvals, bvecs, data, vox_offset=0, layout=LAYOUT_VOX_MAJOR, **kw):
+        o = default_opts(**kw)
+        bv, bvp = _f32(bvals)
+        bc, bcp = _f32(bvecs)
+        d, dp = _f32(data)
+        N = bv.size
+        nvox = d.size // N
+        j = self._chk(self.L.fdt_enqueue_batch(self.h, C.byref(o), N, bvp, bcp, dp, layout, nvox, vox_offset))
+        self.jobs.append((o, nvox))
+        return j
+
+    def run(self):
+        self._chk(self.L.fdt_run_queue(self.h))
+
+    def set_tiles(self, rows_per_wg=0, job=-1, vpl=0):
+        self._chk(self.L.fdt_queue_set_tiles(self.h, rows_per_wg, job, vpl))
+
+    def autotune(self, jumps=50, sample_vox=4096):
+        ms = C.c_float()
+        self._chk(self.L.fdt_queue_autotune(self.h, jumps, sample_vox, C.byref(ms)))
+        return ms.value
+
+    def timing(self):
+        ms = C.c_float()
+        rows = C.c_int()
+        n = len(self.jobs)
+        vpl = (C.c_int * max(n, 1))()
+        self._chk(self.L.fdt_queue_timing(self.h, C.byref(ms), C.byref(rows), vpl, n))
+        return dict(ms=ms.value, rows_per_wg=rows.value, vpl=[vpl[i] for i in range(n)])
+
+    def params(self, job):
+        o, nvox = self.jobs[job]
+        np_ = 2 + 3 * o.nfibres + (1 if o.model >= 2 else 0) + (1 if o.f0 else 0)
+        p = np.zeros((nvox, np_), np.float32)
+        t = np.zeros(nvox, np.float32)
+        self._chk(self.L.fdt_queue_get_params(self.h, job, p.ctypes.data_as(_FP), t.ctypes.data_as(_FP)))
+        return p, t
+
+    def outputs(self, job, raw_scalars=True):
+        o, nvox = self.jobs[job]
+        arr, out = _alloc_outputs(o, nvox, raw_scalars)
+        self._chk(self.L.fdt_queue_get_outputs(self.h, job, C.byref(out)))
         return arr
 
 

@@ -32,6 +32,7 @@ FDT_LM_DECL(2, 1)
 FDT_LM_DECL(3, 1)
 hipError_t fdt_launch_f0_flag(const F0FlagArgs*, hipStream_t);
 hipError_t fdt_launch_zero_ints(int*, int, hipStream_t);
+hipError_t fdt_launch_mcmc_queue(const QueueArgs*, int rows_per_wg, int max_items, hipStream_t);
 hipError_t fdt_launch_finish(const FinishArgs*, hipStream_t);
 hipError_t fdt_launch_transpose(const float*, float*, int, int, hipStream_t);
 #define FDT_VPL_DECL(V) hipError_t fdt_launch_mcmc_vpl##V(const McmcArgs*, int, int, hipStream_t);
@@ -150,6 +151,9 @@ struct fdt_ctx {
   DevBuf<float> d_bvals, d_bvecs;
   DevBuf<double> d_pinv;
   DevBuf<float4> d_acq;
+  DevBuf<int> d_shell;
+  DevBuf<float> d_shell_b;
+  int nshell = 0;
   int Npad = 0;
   int nvox = 0;
   DevBuf<float> d_data, d_stage, d_params, d_tau, d_energy;
@@ -164,6 +168,9 @@ struct fdt_ctx {
   hipEvent_t ev[6] = {};
   int vpl_force = 0, vpl_used = 0;
   bool timed_fit = false, timed_mcmc = false, timed_finish = false;
+  bool quiet = false;  // no timing events (launches recorded into a graph)
+  uint64_t vox_offset = 0;
+  hipEvent_t ev_done = nullptr;
 };
 
 static int set_err(fdt_ctx* c, int code, const char* fmt, ...) {
@@ -235,12 +242,15 @@ void fdt_destroy(fdt_ctx* c) {
   for (auto* b : fb) b->release();
   c->d_pinv.release();
   c->d_acq.release();
+  c->d_shell.release();
+  c->d_shell_b.release();
   c->d_iters.release();
   c->d_queue.release();
   c->d_list.release();
   c->d_s1.release();
   c->d_refit.release();
   for (auto& e : c->ev) if (e) (void)hipEventDestroy(e);
+  if (c->ev_done) (void)hipEventDestroy(c->ev_done);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -273,6 +283,20 @@ int fdt_set_acquisition(fdt_ctx* c, int N, const float* bvals, const float* bvec
   HIPCHK(c, hipMemcpyAsync(c->d_bvecs.p, bvecs, sizeof(float) * 3 * N, hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipMemcpyAsync(c->d_pinv.p, P.data(), sizeof(double) * 7 * N, hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipMemcpyAsync(c->d_acq.p, acq.data(), sizeof(float4) * c->Npad, hipMemcpyHostToDevice, c->stream));
+  // b-shells: distinct b values (exact), used by the MCMC when there are at most 16
+  std::vector<float> shb;
+  std::vector<int> shell(c->Npad, 0);
+  for (int i = 0; i < N && shb.size() <= 16; ++i) {
+    auto it = std::find(shb.begin(), shb.end(), bvals[i]);
+    if (it == shb.end()) { shb.push_back(bvals[i]); shell[i] = (int)shb.size() - 1; }
+    else shell[i] = (int)(it - shb.begin());
+  }
+  c->nshell = shb.size() <= 16 ? (int)shb.size() : 0;
+  shb.resize(16, 0.0f);
+  HIPCHK(c, c->d_shell.ensure(c->Npad));
+  HIPCHK(c, c->d_shell_b.ensure(16));
+  HIPCHK(c, hipMemcpyAsync(c->d_shell.p, shell.data(), sizeof(int) * c->Npad, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->d_shell_b.p, shb.data(), sizeof(float) * 16, hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->nvox = 0;
   c->have_params = c->have_samples = c->have_finish = false;
@@ -364,7 +388,7 @@ static int fit_async(fdt_ctx* c) {
   int* q = c->d_queue.p;
   const int k1 = o.model == 1 ? (o.nonlin ? KIND_SINGLE : KIND_SINGLE_C) : KIND_SINGLE;
   const int G = o.model == 3 ? 2 : 0;
-  HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
+  if (!c->quiet) HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
   HIPCHK(c, fdt_launch_zero_ints(q, NQUEUE, c->stream));
   LmArgs a = base_lm_args(c);
   a.rician = o.rician;
@@ -440,8 +464,8 @@ static int fit_async(fdt_ctx* c) {
     t.tau = c->d_tau.p;
     HIPCHK(c, launch_tau(t, K, fp64, c->stream));
   }
-  HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
-  c->timed_fit = true;
+  if (!c->quiet) HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
+  c->timed_fit = !c->quiet;
   c->have_params = true;
   c->have_samples = c->have_finish = false;
   return FDT_OK;
@@ -543,7 +567,8 @@ int fdt_set_params(fdt_ctx* c, const float* params, const float* tau) {
   return FDT_OK;
 }
 
-static int mcmc_async(fdt_ctx* c, uint64_t vox_offset) {
+// buffers, tile and arguments of the chain launch (no launch)
+static int mcmc_prepare(fdt_ctx* c, uint64_t vox_offset, McmcArgs* out, mcmc_launch_fn* fn_out) {
   if (!c->have_params) return set_err(c, FDT_ESTATE, "no parameters: call fdt_fit or fdt_set_params");
   const fdt_opts& o = c->o;
   int K = o.nfibres, ns = nsamples_of(o);
@@ -570,6 +595,9 @@ static int mcmc_async(fdt_ctx* c, uint64_t vox_offset) {
   a.params = c->d_params.p;
   a.tau = o.rician ? c->d_tau.p : nullptr;
   a.acq = c->d_acq.p;
+  a.shell = c->d_shell.p;
+  a.shell_b = c->d_shell_b.p;
+  a.nshell = c->nshell;
   a.N = c->N;
   a.Npad = 16 * vpl;
   a.nvox = c->nvox;
@@ -597,10 +625,20 @@ static int mcmc_async(fdt_ctx* c, uint64_t vox_offset) {
   a.f0s = o.f0 ? c->s_f0.p : nullptr;
   a.taus = o.rician ? c->s_tau.p : nullptr;
   a.energy = c->d_energy.p;
-  HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
-  HIPCHK(c, fn(&a, K, o.rician, c->stream));
-  HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
-  c->timed_mcmc = true;
+  *out = a;
+  *fn_out = fn;
+  return FDT_OK;
+}
+
+static int mcmc_async(fdt_ctx* c, uint64_t vox_offset) {
+  McmcArgs a;
+  mcmc_launch_fn fn = nullptr;
+  int rc = mcmc_prepare(c, vox_offset, &a, &fn);
+  if (rc) return rc;
+  if (!c->quiet) HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
+  HIPCHK(c, fn(&a, c->o.nfibres, c->o.rician, c->stream));
+  if (!c->quiet) HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
+  c->timed_mcmc = !c->quiet;
   c->have_samples = true;
   c->have_finish = false;
   return FDT_OK;
@@ -638,10 +676,10 @@ static int finish_async(fdt_ctx* c) {
   a.oth = c->o_th.p; a.oph = c->o_ph.p; a.of = c->o_f.p;
   a.mean_S0 = c->m_S0.p; a.mean_d = c->m_d.p; a.mean_dstd = c->m_dstd.p; a.mean_R = c->m_R.p;
   a.mean_f0 = c->m_f0.p; a.mean_tau = c->m_tau.p; a.mean_f = c->m_f.p; a.dyads = c->m_dyads.p; a.disp = c->m_disp.p;
-  HIPCHK(c, hipEventRecord(c->ev[4], c->stream));
+  if (!c->quiet) HIPCHK(c, hipEventRecord(c->ev[4], c->stream));
   if (ns > 0) HIPCHK(c, fdt_launch_finish(&a, c->stream));
-  HIPCHK(c, hipEventRecord(c->ev[5], c->stream));
-  c->timed_finish = true;
+  if (!c->quiet) HIPCHK(c, hipEventRecord(c->ev[5], c->stream));
+  c->timed_finish = !c->quiet;
   c->have_finish = true;
   return FDT_OK;
 }
@@ -723,6 +761,334 @@ int fdt_set_mcmc_tile(fdt_ctx* c, int vpl) {
   if (!c) return FDT_EINVAL;
   if (vpl < 0) return set_err(c, FDT_EINVAL, "vpl=%d", vpl);
   c->vpl_force = vpl;
+  return FDT_OK;
+}
+
+// ------------------------------------------------------------------ queue
+// A batch of volumes run together: every volume's LM on its own stream
+// (forked from and joined to the queue stream), then one persistent chain
+// launch over all volumes whose tile the queue kernel carries, then the
+// finishing passes.  The first fdt_run_queue runs eagerly (allocating and
+// uploading everything) and records the same launches into a hipGraph that
+// later runs replay.
+
+static int tile_of(int N, int force) {
+  for (const auto& e : kVpl)
+    if (16 * e.vpl >= N && e.vpl >= force) return e.vpl;
+  return 0;
+}
+
+}  // extern "C"
+
+struct fdt_queue {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::vector<fdt_ctx*> jobs;
+  std::vector<QueueJob> h_jobs;
+  std::vector<int> h_first;
+  int rows_per_wg = 16;
+  DevBuf<QueueJob> d_jobs, d_tune;
+  DevBuf<int> d_first, d_counter;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipEvent_t ev_fork = nullptr, t0 = nullptr, t1 = nullptr;
+  std::string err;
+  float ms_last = 0.0f;
+  bool ran = false;
+};
+
+static int qerr(fdt_queue* q, int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  if (q) q->err = buf;
+  return code;
+}
+#define QHIP(q, expr)                                                                              \
+  do {                                                                                             \
+    hipError_t e_ = (expr);                                                                        \
+    if (e_ != hipSuccess) return qerr(q, FDT_EHIP, "%s: %s", #expr, hipGetErrorString(e_));        \
+  } while (0)
+#define QCTX(q, c, expr)                                                                           \
+  do {                                                                                             \
+    int rc_ = (expr);                                                                              \
+    if (rc_) return qerr(q, rc_, "%s", (c)->err.c_str());                                          \
+  } while (0)
+
+static void queue_invalidate(fdt_queue* q) {
+  if (q->exec) (void)hipGraphExecDestroy(q->exec);
+  if (q->graph) (void)hipGraphDestroy(q->graph);
+  q->exec = nullptr;
+  q->graph = nullptr;
+}
+
+static bool in_queue_kernel(const fdt_ctx* c) {
+  int vpl = tile_of(c->N, c->vpl_force);
+  return !c->o.rician && vpl > 0 && vpl <= QUEUE_MAX_VPL;
+}
+
+static int queue_launch_chains(fdt_queue* q, const QueueJob* d_jobs, int njobs, const int* d_first, int total) {
+  QueueArgs qa{};
+  qa.jobs = d_jobs;
+  qa.first_item = d_first;
+  qa.njobs = njobs;
+  qa.counter = q->d_counter.p;
+  QHIP(q, fdt_launch_zero_ints(q->d_counter.p, 1, q->stream));
+  QHIP(q, fdt_launch_mcmc_queue(&qa, q->rows_per_wg, total, q->stream));
+  return FDT_OK;
+}
+
+// issue the whole batch on the queue stream (upload: also rebuild and copy the job table)
+static int queue_issue(fdt_queue* q, bool upload) {
+  QHIP(q, hipEventRecord(q->ev_fork, q->stream));
+  for (fdt_ctx* c : q->jobs) {
+    QHIP(q, hipStreamWaitEvent(c->stream, q->ev_fork, 0));
+    QCTX(q, c, fit_async(c));
+    if (!in_queue_kernel(c)) QCTX(q, c, mcmc_async(c, c->vox_offset));
+    QHIP(q, hipEventRecord(c->ev_done, c->stream));
+    QHIP(q, hipStreamWaitEvent(q->stream, c->ev_done, 0));
+  }
+  if (upload) {
+    q->h_jobs.clear();
+    q->h_first.assign(1, 0);
+    for (fdt_ctx* c : q->jobs) {
+      if (!in_queue_kernel(c)) continue;
+      QueueJob j{};
+      mcmc_launch_fn fn = nullptr;
+      QCTX(q, c, mcmc_prepare(c, c->vox_offset, &j.a, &fn));
+      j.K = c->o.nfibres;
+      j.vpl = c->vpl_used;
+      q->h_jobs.push_back(j);
+      q->h_first.push_back(q->h_first.back() + (c->nvox + q->rows_per_wg - 1) / q->rows_per_wg);
+    }
+    if (!q->h_jobs.empty()) {
+      QHIP(q, q->d_jobs.ensure(q->h_jobs.size()));
+      QHIP(q, q->d_first.ensure(q->h_first.size()));
+      QHIP(q, hipMemcpyAsync(q->d_jobs.p, q->h_jobs.data(), sizeof(QueueJob) * q->h_jobs.size(), hipMemcpyHostToDevice,
+                             q->stream));
+      QHIP(q, hipMemcpyAsync(q->d_first.p, q->h_first.data(), sizeof(int) * q->h_first.size(), hipMemcpyHostToDevice,
+                             q->stream));
+    }
+  }
+  if (!q->h_jobs.empty())
+    if (int rc = queue_launch_chains(q, q->d_jobs.p, (int)q->h_jobs.size(), q->d_first.p, q->h_first.back())) return rc;
+  QHIP(q, hipEventRecord(q->ev_fork, q->stream));
+  for (fdt_ctx* c : q->jobs) {
+    c->have_samples = true;
+    QHIP(q, hipStreamWaitEvent(c->stream, q->ev_fork, 0));
+    QCTX(q, c, finish_async(c));
+    QHIP(q, hipEventRecord(c->ev_done, c->stream));
+    QHIP(q, hipStreamWaitEvent(q->stream, c->ev_done, 0));
+  }
+  return FDT_OK;
+}
+
+extern "C" {
+
+fdt_queue* fdt_queue_create(int device) {
+  fdt_queue* q = new fdt_queue;
+  q->device = device;
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&q->stream) != hipSuccess ||
+      hipEventCreateWithFlags(&q->ev_fork, hipEventDisableTiming) != hipSuccess || hipEventCreate(&q->t0) != hipSuccess ||
+      hipEventCreate(&q->t1) != hipSuccess || q->d_counter.ensure(1) != hipSuccess) {
+    delete q;
+    return nullptr;
+  }
+  return q;
+}
+
+void fdt_queue_destroy(fdt_queue* q) {
+  if (!q) return;
+  (void)hipSetDevice(q->device);
+  if (q->stream) (void)hipStreamSynchronize(q->stream);
+  queue_invalidate(q);
+  for (fdt_ctx* c : q->jobs) fdt_destroy(c);
+  q->d_jobs.release();
+  q->d_tune.release();
+  q->d_first.release();
+  q->d_counter.release();
+  if (q->ev_fork) (void)hipEventDestroy(q->ev_fork);
+  if (q->t0) (void)hipEventDestroy(q->t0);
+  if (q->t1) (void)hipEventDestroy(q->t1);
+  if (q->stream) (void)hipStreamDestroy(q->stream);
+  delete q;
+}
+
+const char* fdt_queue_last_error(const fdt_queue* q) { return q ? q->err.c_str() : "null queue"; }
+
+int fdt_enqueue_batch(fdt_queue* q, const fdt_opts* o, int N, const float* bvals, const float* bvecs, const float* data,
+                      int layout, int nvox, uint64_t vox_offset) {
+  if (!q || !o) return FDT_EINVAL;
+  (void)hipSetDevice(q->device);
+  fdt_ctx* c = fdt_create(q->device, o);
+  if (!c) return qerr(q, FDT_EINVAL, "job %zu: invalid options or device", q->jobs.size());
+  int rc = fdt_set_acquisition(c, N, bvals, bvecs);
+  if (!rc) rc = fdt_load_data(c, data, layout, nvox);
+  if (!rc && hipEventCreateWithFlags(&c->ev_done, hipEventDisableTiming) != hipSuccess) rc = FDT_EHIP;
+  if (rc) {
+    qerr(q, rc, "job %zu: %s", q->jobs.size(), c->err.c_str());
+    fdt_destroy(c);
+    return rc;
+  }
+  c->vox_offset = vox_offset;
+  c->quiet = true;
+  q->jobs.push_back(c);
+  queue_invalidate(q);
+  q->ran = false;
+  return (int)q->jobs.size() - 1;
+}
+
+int fdt_queue_set_tiles(fdt_queue* q, int rows_per_wg, int job, int vpl) {
+  if (!q) return FDT_EINVAL;
+  if (rows_per_wg) {
+    if (rows_per_wg != 4 && rows_per_wg != 8 && rows_per_wg != 16)
+      return qerr(q, FDT_EINVAL, "rows_per_wg=%d: must be 4, 8 or 16", rows_per_wg);
+    q->rows_per_wg = rows_per_wg;
+  }
+  if (job >= 0) {
+    if (job >= (int)q->jobs.size()) return qerr(q, FDT_EINVAL, "job %d out of range", job);
+    if (vpl < 0 || (vpl && !tile_of(q->jobs[job]->N, vpl))) return qerr(q, FDT_EINVAL, "vpl=%d", vpl);
+    q->jobs[job]->vpl_force = vpl;
+  }
+  queue_invalidate(q);
+  q->ran = false;
+  return FDT_OK;
+}
+
+int fdt_run_queue(fdt_queue* q) {
+  if (!q) return FDT_EINVAL;
+  if (q->jobs.empty()) return qerr(q, FDT_ESTATE, "empty queue");
+  (void)hipSetDevice(q->device);
+  QHIP(q, hipEventRecord(q->t0, q->stream));
+  if (!q->exec) {
+    if (int rc = queue_issue(q, true)) return rc;
+    QHIP(q, hipEventRecord(q->t1, q->stream));
+    QHIP(q, hipStreamSynchronize(q->stream));
+    QHIP(q, hipEventElapsedTime(&q->ms_last, q->t0, q->t1));
+    // record the same launches into a graph for the next runs
+    QHIP(q, hipStreamBeginCapture(q->stream, hipStreamCaptureModeRelaxed));
+    int rc = queue_issue(q, false);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(q->stream, &g);
+    if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
+    QHIP(q, e);
+    q->graph = g;
+    QHIP(q, hipGraphInstantiate(&q->exec, q->graph, nullptr, nullptr, 0));
+  } else {
+    QHIP(q, hipGraphLaunch(q->exec, q->stream));
+    QHIP(q, hipEventRecord(q->t1, q->stream));
+    QHIP(q, hipStreamSynchronize(q->stream));
+    QHIP(q, hipEventElapsedTime(&q->ms_last, q->t0, q->t1));
+  }
+  q->ran = true;
+  return FDT_OK;
+}
+
+int fdt_queue_num_jobs(const fdt_queue* q) { return q ? (int)q->jobs.size() : FDT_EINVAL; }
+
+int fdt_queue_get_outputs(fdt_queue* q, int job, const fdt_outputs* out) {
+  if (!q || !out) return FDT_EINVAL;
+  if (job < 0 || job >= (int)q->jobs.size()) return qerr(q, FDT_EINVAL, "job %d out of range", job);
+  if (!q->ran) return qerr(q, FDT_ESTATE, "fdt_run_queue has not run since the last change");
+  QHIP(q, hipStreamSynchronize(q->stream));
+  QCTX(q, q->jobs[job], fdt_get_outputs(q->jobs[job], out));
+  return FDT_OK;
+}
+
+int fdt_queue_get_params(fdt_queue* q, int job, float* params, float* tau) {
+  if (!q) return FDT_EINVAL;
+  if (job < 0 || job >= (int)q->jobs.size()) return qerr(q, FDT_EINVAL, "job %d out of range", job);
+  QHIP(q, hipStreamSynchronize(q->stream));
+  QCTX(q, q->jobs[job], fdt_get_params(q->jobs[job], params, tau));
+  return FDT_OK;
+}
+
+int fdt_queue_timing(fdt_queue* q, float* ms_last, int* rows_per_wg, int* vpl_of_job, int njobs) {
+  if (!q) return FDT_EINVAL;
+  if (ms_last) *ms_last = q->ms_last;
+  if (rows_per_wg) *rows_per_wg = q->rows_per_wg;
+  if (vpl_of_job)
+    for (int j = 0; j < njobs && j < (int)q->jobs.size(); ++j) vpl_of_job[j] = tile_of(q->jobs[j]->N, q->jobs[j]->vpl_force);
+  return FDT_OK;
+}
+
+// Autotuner: times the queue kernel on a sample of every volume (first
+// `sample_vox` voxels, `jumps` sweeps, no burn-in) for each voxels-per-
+// workgroup choice (4, 8, 16), keeps the fastest, then for each volume tries
+// the next larger volumes-per-lane tile and keeps it when faster.  Needs LM
+// parameters: the first call runs the batch once.
+int fdt_queue_autotune(fdt_queue* q, int jumps, int sample_vox, float* ms_best) {
+  if (!q) return FDT_EINVAL;
+  if (q->jobs.empty()) return qerr(q, FDT_ESTATE, "empty queue");
+  if (jumps <= 0 || sample_vox <= 0) return qerr(q, FDT_EINVAL, "jumps=%d sample_vox=%d", jumps, sample_vox);
+  (void)hipSetDevice(q->device);
+  if (!q->ran)
+    if (int rc = fdt_run_queue(q)) return rc;
+  auto time_config = [&](float* ms) -> int {
+    std::vector<QueueJob> tj;
+    std::vector<int> first(1, 0);
+    for (fdt_ctx* c : q->jobs) {
+      if (!in_queue_kernel(c)) continue;
+      QueueJob j{};
+      mcmc_launch_fn fn = nullptr;
+      QCTX(q, c, mcmc_prepare(c, c->vox_offset, &j.a, &fn));
+      j.K = c->o.nfibres;
+      j.vpl = c->vpl_used;
+      j.a.nvox = std::min(c->nvox, sample_vox);
+      j.a.nburn = 0;
+      j.a.njumps = jumps;
+      j.a.se = jumps;
+      j.a.nsamples = 1;  // writes sample 0 of the volume's buffers (stride nvox kept)
+      j.a.energy = nullptr;
+      tj.push_back(j);
+      first.push_back(first.back() + (j.a.nvox + q->rows_per_wg - 1) / q->rows_per_wg);
+    }
+    if (tj.empty()) { *ms = 0.0f; return FDT_OK; }
+    QHIP(q, q->d_tune.ensure(tj.size()));
+    DevBuf<int> dfirst;
+    QHIP(q, dfirst.ensure(first.size()));
+    QHIP(q, hipMemcpyAsync(q->d_tune.p, tj.data(), sizeof(QueueJob) * tj.size(), hipMemcpyHostToDevice, q->stream));
+    QHIP(q, hipMemcpyAsync(dfirst.p, first.data(), sizeof(int) * first.size(), hipMemcpyHostToDevice, q->stream));
+    float best = 1e30f;
+    for (int rep = 0; rep < 3; ++rep) {
+      QHIP(q, hipEventRecord(q->t0, q->stream));
+      if (int rc = queue_launch_chains(q, q->d_tune.p, (int)tj.size(), dfirst.p, first.back())) return rc;
+      QHIP(q, hipEventRecord(q->t1, q->stream));
+      QHIP(q, hipStreamSynchronize(q->stream));
+      float ms = 0.0f;
+      QHIP(q, hipEventElapsedTime(&ms, q->t0, q->t1));
+      if (rep > 0) best = std::min(best, ms);
+    }
+    dfirst.release();
+    *ms = best;
+    return FDT_OK;
+  };
+  float best = 1e30f;
+  int best_rows = q->rows_per_wg;
+  for (int rows : {4, 8, 16}) {
+    q->rows_per_wg = rows;
+    float ms;
+    if (int rc = time_config(&ms)) return rc;
+    if (ms < best) { best = ms; best_rows = rows; }
+  }
+  q->rows_per_wg = best_rows;
+  for (fdt_ctx* c : q->jobs) {
+    if (!in_queue_kernel(c)) continue;
+    int cur = tile_of(c->N, c->vpl_force);
+    int next = tile_of(c->N, cur + 1);
+    if (!next || next > QUEUE_MAX_VPL) continue;
+    int keep = c->vpl_force;
+    c->vpl_force = next;
+    float ms;
+    if (int rc = time_config(&ms)) return rc;
+    if (ms < best) best = ms;
+    else c->vpl_force = keep;
+  }
+  if (ms_best) *ms_best = best;
+  queue_invalidate(q);
+  q->ran = false;  // the timing runs overwrote sample 0 of each volume
   return FDT_OK;
 }
 

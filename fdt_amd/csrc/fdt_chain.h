@@ -1,0 +1,744 @@
+// fdt_chain.h -- one Metropolis-Hastings chain of the ball-and-sticks models
+// per 16-lane row on gfx950, shared by the per-volume kernel (fdt_mcmc.hip)
+// and the batched queue kernel (fdt_queue.hip).
+//
+// Semantics follow the reference GPU MCMC: init_Fibres_Multifibres_kernel
+// (CUDA/runmcmc_kernels.cu:193-505) and runmcmc_kernel (:507-1474), with the
+// burn-in and record launches (CUDA/runmcmc.cu:108-316) fused into one loop of
+// nburn + njumps sweeps.  Layout: one voxel per 16-lane DPP row, four voxels
+// per wave64.  Lane l of a row holds volumes l, l+16, l+32, ... (VPL volumes
+// per lane) and keeps, in VGPRs,
+//   the residual r = y - S0*M (Gaussian) or the data y (Rician),
+//   the mixture M = sum_k f_k s_k + (1 - f0 - sum_k f_k) iso + f0,
+//   the isotropic signal iso and the stick signals s_k;
+// b*(g.x_k)^2, read only by the diffusivity proposals, sits in LDS (one block
+// per wave, lane-interleaved), so that K=3 at 18 volumes per lane fits two
+// waves per SIMD.
+// A proposal evaluates only what it changes; its likelihood change is
+// summed over the row with DPP adds.  For Gaussian noise a move of every
+// residual by -c*q gives the change of the sum of squares c*(c*sum q^2 -
+// 2*sum r*q), and the energy difference (N/2)*log1p(dSSR/SSR) keeps full
+// relative precision (the reference differences two FP32 energies).  The
+// acquisition (gx,gy,gz,b) is staged in LDS.  Draws: lane l of the row
+// computes the Philox block of proposal slot l for the current jump; DPP
+// row_newbcast hands it to the row.
+#pragma once
+#include "fdt_device.h"
+#include "fdt_kernels.h"
+#include <utility>
+
+namespace fdt {
+
+constexpr float L2E = 1.4426950408889634f;
+constexpr float LN2 = 0.6931471805599453f;
+FDT_DEV float ex2(float x) { return __builtin_amdgcn_exp2f(x); }
+FDT_DEV float lg2(float x) { return __builtin_amdgcn_logf(x); }
+FDT_DEV float flog(float x) { return lg2(x) * LN2; }
+
+// draw slots (fixed per parameter)
+enum { SL_F0 = 0, SL_TAU = 1, SL_D = 2, SL_DSTD = 3, SL_R = 4, SL_S0 = 5, SL_FIB = 6 };
+
+// Signal forms with row-uniform coefficients (compute_signal /
+// compute_iso_signal, CUDA/runmcmc_kernels.cu:67-95):
+//   kind 0: exp(-d*u)                        model 1, or d_std < 1e-5
+//   kind 1: (1 + (dstd^2/d) u)^(-d^2/dstd^2)  model 2 (gamma diffusivities)
+//   kind 2: exp(-3 d invR ((1-R) b a + R b))  model 3 zeppelin, u = b a
+struct SigC {
+  int kind;
+  float c0, c1;
+};
+FDT_DEV SigC aniso_coef(int model, float d, float dstd, float R) {
+  SigC c;
+  if (model == 1 || (model == 2 && dstd < 1e-5f)) { c.kind = 0; c.c0 = -d * L2E; c.c1 = 0.0f; }
+  else if (model == 2) { float s2 = dstd * dstd; c.kind = 1; c.c0 = s2 / d; c.c1 = -(d * d / s2); }
+  else {
+    float iR = 1.0f / (2.0f * R + 1.0f);
+    float k = -3.0f * d * iR * L2E;
+    c.kind = 2; c.c0 = k * (1.0f - R); c.c1 = k * R;
+  }
+  return c;
+}
+FDT_DEV SigC iso_coef(int model, float d, float dstd) {
+  SigC c;
+  if (model == 1 || dstd < 1e-5f) { c.kind = 0; c.c0 = -d * L2E; c.c1 = 0.0f; }
+  else { float s2 = dstd * dstd; c.kind = 1; c.c0 = s2 / d; c.c1 = -(d * d / s2); }
+  return c;
+}
+template <int SK>
+FDT_DEV float sig(const SigC& c, float u, float b) {
+  if constexpr (SK == 0) return ex2(c.c0 * u);
+  else if constexpr (SK == 1) return ex2(c.c1 * lg2(fmaf(c.c0, u, 1.0f)));
+  else return ex2(fmaf(c.c0, u, c.c1 * b));
+}
+template <class F>
+FDT_DEV void by_kind(int kind, F&& f) {
+  if (kind == 0) f(std::integral_constant<int, 0>{});
+  else if (kind == 1) f(std::integral_constant<int, 1>{});
+  else f(std::integral_constant<int, 2>{});
+}
+
+// log I0 by Abramowitz-Stegun 9.8.1/9.8.2 (fibre.h:29-48, CUDA/runmcmc_kernels.cu:126-145)
+FDT_DEV float logI0f(float x) {
+  float b = fabsf(x);
+  float a = x * (1.0f / 3.75f);
+  a *= a;
+  float ys = 1.0f + a * (3.5156229f + a * (3.0899424f + a * (1.2067492f + a * (0.2659732f + a * (0.0360768f + a * 0.0045813f)))));
+  float Ls = flog(ys);
+  float c = 3.75f * __builtin_amdgcn_rcpf(b);
+  float pl = 0.39894228f + c * (0.01328592f + c * (0.00225319f + c * (-0.00157565f + c * (0.00916281f + c * (-0.02057706f + c * (0.02635537f + c * (-0.01647633f + c * 0.00392377f)))))));
+  float Ll = b + flog(pl * __builtin_amdgcn_rsqf(b));
+  return b < 3.75f ? Ls : Ll;
+}
+// Rician log-likelihood term of one volume (fibre.h:949-953)
+FDT_DEV float ell_of(float y, float p, float tau) {
+  return flog(y) - 0.5f * tau * (y * y + p * p) + logI0f(tau * p * y);
+}
+
+// Gaussian likelihood change when every residual moves by -c*q_t:
+// sum (r - c q)^2 - r^2 = c (c sum q^2 - 2 sum r q); two FMAs per volume and
+// two row sums
+struct Quad {
+  float A = 0.0f, B = 0.0f;
+  FDT_DEV void add(float r, float q) { A = fmaf(r, q, A); B = fmaf(q, q, B); }
+  FDT_DEV float delta(float c) {
+    A = row_sum16(A);
+    B = row_sum16(B);
+    return c * fmaf(c, B, -2.0f * A);
+  }
+};
+// Compiler-only memory barrier: keeps the acquisition table in LDS (re-read
+// per proposal) instead of letting the loop-invariant loads be hoisted into
+// ~4*VPL VGPRs for the whole chain.
+FDT_DEV void reload_acq() { asm volatile("" ::: "memory"); }
+FDT_DEV int acc1(int c) { return c + 1; }
+FDT_DEV int rej1(int c) { return c + (1 << 16); }
+
+// compile-time loop: f(integral_constant<int, i>) for i = 0..N-1
+template <int N, class F>
+FDT_DEV void sfor(F&& f) {
+  [&]<int... is>(std::integer_sequence<int, is...>) { (f(std::integral_constant<int, is>{}), ...); }(
+      std::make_integer_sequence<int, N>{});
+}
+
+// One chain per 16-lane row.  Volume state lives in registers per lane;
+// everything that belongs to a proposal slot -- the parameter value, its
+// proposal width, its acceptance counters (acc + rej << 16) and its current
+// prior energy -- lives in the lane of that slot (sval/sprop/scnt/sprior) and
+// is read by a row broadcast.  Slots: f0 0, tau 1, d 2, d_std 3, R 4, S0 5,
+// theta_k 6+3k, phi_k 7+3k, f_k 8+3k (the theta prior sits in the theta slot).
+template <int K, int VPL, bool RIC>
+struct Chain {
+  // lane-resident volume state, volume i = rl + 16 t
+  float r[VPL];  // Gaussian: y - S0*M; Rician: y
+  float M[VPL];
+  float iso[VPL];
+  float s[K][VPL];
+  float* bal;  // b*(g.x_k)^2 of the lane's volumes, in the wave's LDS block: bal[(k*VPL+t)*64]
+  float ell[RIC ? VPL : 1];
+  // isotropic signal per b-shell (acquisitions with <= 16 distinct b values):
+  // sh[i] is the shell of volume i, shb[l] its b value, isot the row's table
+  const int* sh;
+  const float* shb;
+  float* isot;
+  int nshell;
+  float sval, sprop, sprior;
+  int scnt;
+  double like;  // Gaussian: SSR; Rician: sum of ell
+  bool last_ok;  // the lane's last slot holds a real volume
+  int rl;
+  // options
+  int model;
+  bool incf0, ardf0, can_ard;
+  bool lam[K];
+  float fudge, halfN, Nf, Rmean, Rstd, Rfudge;
+  const float4* acq;
+
+  static constexpr int TH(int k) { return SL_FIB + 3 * k; }
+  static constexpr int PH(int k) { return SL_FIB + 3 * k + 1; }
+  static constexpr int FK(int k) { return SL_FIB + 3 * k + 2; }
+  template <int SL> FDT_DEV float v() const { return row_bcast<SL>(sval); }
+  template <int SL> FDT_DEV float width() const { return row_bcast<SL>(sprop); }
+  template <int SL> FDT_DEV float prior() const { return row_bcast<SL>(sprior); }
+  template <int SL> FDT_DEV void reject() { if (rl == SL) scnt += 1 << 16; }
+  template <int SL> FDT_DEV void accept(float val, float prn) {
+    if (rl == SL) { sval = val; scnt += 1; sprior = prn; }
+  }
+
+  FDT_DEV float ba(int k, int t) const { return bal[(k * VPL + t) * 64]; }
+  FDT_DEV float mask_last(int t, float x) const { return (t == VPL - 1 && !last_ok) ? 0.0f : x; }
+  // the fractions of the row: fk[k] and f0, their sum
+  struct Fr {
+    float fk[K];
+    float f0, sum;
+  };
+  FDT_DEV Fr fractions() const {
+    Fr q;
+    q.f0 = v<SL_F0>();
+    q.sum = q.f0;
+    sfor<K>([&](auto kc) { q.fk[kc] = v<FK(kc)>(); q.sum += q.fk[kc]; });
+    return q;
+  }
+  // energy change of the likelihood from a row-summed change `dl`
+  FDT_DEV float dlik(float dl) const {
+    if constexpr (RIC) return -dl;
+    else {
+      // log1p(x) as log(u) * x / (u - 1) with u = 1 + x (exact when u == 1)
+      float x = dl * __builtin_amdgcn_rcpf((float)like);
+      float u = 1.0f + x;
+      float lp = (u == 1.0f) ? x : flog(u) * (x * __builtin_amdgcn_rcpf(u - 1.0f));
+      return halfN * lp;
+    }
+  }
+  FDT_DEV static float theta_prior(float t) {  // fibre.h:283-290
+    return t == 0.0f ? 0.0f : -flog(fabsf(__sinf(t)) * 0.5f);
+  }
+  // R prior (fibre.h:816-843) at diffusivity d; false outside its support
+  FDT_DEV bool r_prior(float Rv, float d, float& pr) const {
+    float upper = 2.0f * Rmean, lower = Rmean - 2.0f * Rstd;
+    if (Rmean > 0.5f) upper = 1.0f;
+    if (lower < 0.0f) lower = 1e-8f;
+    if (Rfudge > 0.0f && d > UPPERDIFF_F / 2.0f) {
+      if (Rv < 1e-8f || Rv > upper) return false;
+      pr = Rfudge * flog(Rv);
+      return true;
+    }
+    if (Rv <= lower || Rv > upper) return false;
+    pr = (Rv - Rmean) * (Rv - Rmean) / (Rstd * Rstd);
+    return true;
+  }
+  // mixture of slot t from the stored signals
+  FDT_DEV float mix(int t, const Fr& q) const {
+    float m = fmaf(1.0f - q.sum, iso[t], q.f0);
+#pragma unroll
+    for (int k = 0; k < K; ++k) m = fmaf(q.fk[k], s[k][t], m);
+    return m;
+  }
+
+  // rebuild M and the residual (or the Rician terms) from the stored signals
+  FDT_DEV void rebuild(const float* y) {
+    const Fr q = fractions();
+    const float S0 = v<SL_S0>(), tau = v<SL_TAU>();
+    double ss = 0.0;
+#pragma unroll
+    for (int t = 0; t < VPL; ++t) {
+      float m = mix(t, q);
+      M[t] = m;
+      if constexpr (RIC) {
+        ell[t] = ell_of(r[t], S0 * m, tau);
+        ss += (double)mask_last(t, ell[t]);
+      } else {
+        r[t] = y[t] - S0 * m;
+        ss += (double)mask_last(t, r[t] * r[t]);
+      }
+    }
+    like = (double)row_sum16((float)ss);
+  }
+
+  // ---------------------------------------------------------- proposals
+  // n: standard normal draw, lu: log of the uniform draw of the slot.
+  // f0 (CUDA/runmcmc_kernels.cu:842-888)
+  FDT_DEV void prop_f0(float n, float lu) {
+    const float f0 = v<SL_F0>();
+    const float f0n = f0 + n * width<SL_F0>();
+    const Fr q = fractions();
+    const float fsn = q.sum - f0 + f0n;
+    if (f0n <= 0.0f || f0n >= 1.0f || fsn > 1.0f) { reject<SL_F0>(); return; }
+    const float prn = ardf0 ? flog(f0n) : 0.0f;
+    const float df = f0n - f0, S0 = v<SL_S0>(), tau = v<SL_TAU>();
+    float a = 0.0f, en[RIC ? VPL : 1];
+    Quad qd;
+#pragma unroll
+    for (int t = 0; t < VPL; ++t) {
+      if constexpr (RIC) {
+        en[t] = ell_of(r[t], S0 * fmaf(df, 1.0f - iso[t], M[t]), tau);
+        a += mask_last(t, en[t] - ell[t]);
+      } else qd.add(r[t], mask_last(t, 1.0f - iso[t]));
+    }
+    a = RIC ? row_sum16(a) : qd.delta(S0 * df);
+    if (dlik(a) + (prn - prior<SL_F0>()) < -lu) {
+#pragma unroll
+      for (int t = 0; t < VPL; ++t) {
+        float dM = df * (1.0f - iso[t]);
+        M[t] += dM;
+        if constexpr (RIC) ell[t] = en[t]; else r[t] -= S0 * dM;
+      }
+      like += (double)a;
+      accept<SL_F0>(f0n, prn);
+    } else reject<SL_F0>();
+  }
+  // tau (:890-922), Rician only
+  FDT_DEV void prop_tau(float n, float lu) {
+    if constexpr (RIC) {
+      const float tau = v<SL_TAU>();
+      const float tn = tau + n * width<SL_TAU>();
+      if (tn <= 0.0f) { reject<SL_TAU>(); return; }
+      const float S0 = v<SL_S0>();
+      float a = 0.0f, en[VPL];
+#pragma unroll
+      for (int t = 0; t < VPL; ++t) { en[t] = ell_of(r[t], S0 * M[t], tn); a += mask_last(t, en[t] - ell[t]); }
+      a = row_sum16(a);
+      float dE = -Nf * (flog(tn) - flog(tau)) - a;
+      if (dE < -lu) {
+#pragma unroll
+        for (int t = 0; t < VPL; ++t) ell[t] = en[t];
+        like += (double)a;
+        accept<SL_TAU>(tn, 0.0f);
+      } else reject<SL_TAU>();
+    }
+  }
+  // S0 (:1125-1155)
+  FDT_DEV void prop_S0(float n, float lu) {
+    const float S0 = v<SL_S0>();
+    const float Sn = S0 + n * width<SL_S0>();
+    if (Sn < 0.0f) { reject<SL_S0>(); return; }
+    const float dS = Sn - S0, tau = v<SL_TAU>();
+    float a = 0.0f, en[RIC ? VPL : 1];
+    Quad qd;
+#pragma unroll
+    for (int t = 0; t < VPL; ++t) {
+      if constexpr (RIC) { en[t] = ell_of(r[t], Sn * M[t], tau); a += mask_last(t, en[t] - ell[t]); }
+      else qd.add(r[t], mask_last(t, M[t]));
+    }
+    a = RIC ? row_sum16(a) : qd.delta(dS);
+    if (dlik(a) < -lu) {
+#pragma unroll
+      for (int t = 0; t < VPL; ++t) {
+        if constexpr (RIC) ell[t] = en[t]; else r[t] -= dS * M[t];
+      }
+      like += (double)a;
+      accept<SL_S0>(Sn, 0.0f);
+    } else reject<SL_S0>();
+  }
+  // candidate isotropic signal of every shell into the row's table (lane l: shell l)
+  template <int SI>
+  FDT_DEV void fill_iso(const SigC& ci) const {
+    float b = shb[rl];
+    isot[rl] = sig<SI>(ci, b, b);
+  }
+  // candidate isotropic signal of slot t: from the shell table (SH) or evaluated
+  template <int SI, bool SH>
+  FDT_DEV float iso_new(int t, const SigC& ci, float b) const {
+    if constexpr (SH) return isot[sh[rl + 16 * t]];
+    else return sig<SI>(ci, b, b);
+  }
+  // candidate mixture of slot t when the stick signals (NA) and/or the
+  // isotropic signal (NI) are re-evaluated with coefficients ca / ci
+  template <int SA, int SI, bool NA, bool NI, bool SH>
+  FDT_DEV float mix_new(int t, const Fr& q, const SigC& ca, const SigC& ci) const {
+    float b = acq[rl + 16 * t].w;
+    float isn = NI ? iso_new<SI, SH>(t, ci, b) : iso[t];
+    float m = fmaf(1.0f - q.sum, isn, q.f0);
+#pragma unroll
+    for (int k = 0; k < K; ++k) m = fmaf(q.fk[k], NA ? sig<SA>(ca, ba(k, t), b) : s[k][t], m);
+    return m;
+  }
+  template <int SA, int SI, bool NA, bool NI, bool SH>
+  FDT_DEV void store_new(int t, const SigC& ca, const SigC& ci) {
+    float b = acq[rl + 16 * t].w;
+    if (NI) iso[t] = iso_new<SI, SH>(t, ci, b);
+    if (NA) {
+#pragma unroll
+      for (int k = 0; k < K; ++k) s[k][t] = sig<SA>(ca, ba(k, t), b);
+    }
+  }
+  // dispatch over the coefficient forms that occur: model 1 (0,0); model 2
+  // (1,1) or (0,0) below d_std 1e-5; model 3 sticks 2 with iso 1 or 0
+  // run f(SH) with SH = true when the isotropic signal changes and the
+  // acquisition has a shell table (the loop over slots is then the table
+  // variant only; no per-slot branch)
+  template <bool NI, class F>
+  FDT_DEV void by_shells(F&& f) const {
+    if constexpr (NI) {
+      if (nshell) f(std::true_type{});
+      else f(std::false_type{});
+    } else f(std::false_type{});
+  }
+  template <class F>
+  FDT_DEV void by_forms(const SigC& ca, const SigC& ci, bool na, bool ni, F&& f) const {
+    using I0 = std::integral_constant<int, 0>;
+    using I1 = std::integral_constant<int, 1>;
+    using I2 = std::integral_constant<int, 2>;
+    using T_ = std::true_type;
+    using F_ = std::false_type;
+    if (na && ni) {
+      if (ca.kind == 0) f(I0{}, I0{}, T_{}, T_{});
+      else if (ca.kind == 1) f(I1{}, I1{}, T_{}, T_{});
+      else if (ci.kind == 1) f(I2{}, I1{}, T_{}, T_{});
+      else f(I2{}, I0{}, T_{}, T_{});
+    } else if (ni) {
+      if (ci.kind == 1) f(I0{}, I1{}, F_{}, T_{});
+      else f(I0{}, I0{}, F_{}, T_{});
+    } else {
+      f(I2{}, I0{}, T_{}, F_{});
+    }
+  }
+  // all stick signals and/or iso change: d (:924-985), d_std (:987-1051), R
+  // (:1053-1122).  Neither the candidate signals nor the candidate mixture are
+  // kept: the row sum is accumulated on the fly and everything is re-evaluated
+  // when the move is accepted, which keeps the live registers to the chain
+  // state.
+  template <int SL>
+  FDT_DEV void prop_global(float n, float lu) {
+    const float d = v<SL_D>(), dstd = v<SL_DSTD>(), Rr = v<SL_R>();
+    float dn = d, dsn = dstd, Rn = Rr, prn = 0.0f, xn;
+    bool new_aniso = true, new_iso = true;
+    if constexpr (SL == SL_D) {
+      dn = xn = d + n * width<SL>();
+      if (dn < 0.0f || dn > UPPERDIFF_F) { reject<SL>(); return; }
+      prn = model == 3 ? (1.0f - 3.0f) * flog(dn) + 4000.0f * dn : 0.0f;
+    } else if constexpr (SL == SL_DSTD) {
+      dsn = xn = dstd + n * width<SL>();
+      float up = model == 3 ? 0.004f : 0.01f;
+      if (dsn <= 0.0f || dsn > up) { reject<SL>(); return; }
+      prn = flog(dsn);
+      new_aniso = (model == 2);
+    } else {
+      Rn = xn = Rr + n * width<SL>();
+      if (!r_prior(Rn, d, prn)) { reject<SL>(); return; }
+      new_iso = false;
+    }
+    const SigC ca = aniso_coef(model, dn, dsn, Rn), ci = iso_coef(model, dn, dsn);
+    const Fr q = fractions();
+    const float S0 = v<SL_S0>(), tau = v<SL_TAU>();
+    reload_acq();
+    float a = 0.0f;
+    Quad qd;
+    by_forms(ca, ci, new_aniso, new_iso, [&](auto SA, auto SI, auto NA, auto NI) {
+      by_shells<decltype(NI)::value>([&](auto SH) {
+        if constexpr (decltype(SH)::value) fill_iso<decltype(SI)::value>(ci);
+#pragma unroll
+        for (int t = 0; t < VPL; ++t) {
+          float mn = mix_new<decltype(SA)::value, decltype(SI)::value, decltype(NA)::value, decltype(NI)::value,
+                             decltype(SH)::value>(t, q, ca, ci);
+          if constexpr (RIC) a += mask_last(t, ell_of(r[t], S0 * mn, tau) - ell[t]);
+          else qd.add(r[t], mask_last(t, mn - M[t]));
+        }
+      });
+    });
+    a = RIC ? row_sum16(a) : qd.delta(S0);
+    if (dlik(a) + (prn - prior<SL>()) < -lu) {
+      // store the new signals, then form the mixture from them with the
+      // same expression as the candidate (bit-identical to it)
+      reload_acq();
+      by_forms(ca, ci, new_aniso, new_iso, [&](auto SA, auto SI, auto NA, auto NI) {
+        by_shells<decltype(NI)::value>([&](auto SH) {
+#pragma unroll
+          for (int t = 0; t < VPL; ++t) {
+            store_new<decltype(SA)::value, decltype(SI)::value, decltype(NA)::value, decltype(NI)::value,
+                      decltype(SH)::value>(t, ca, ci);
+            float mn = mix(t, q);
+            if constexpr (RIC) ell[t] = ell_of(r[t], S0 * mn, tau); else r[t] -= S0 * (mn - M[t]);
+            M[t] = mn;
+          }
+        });
+      });
+      like += (double)a;
+      accept<SL>(xn, prn);
+    } else reject<SL>();
+  }
+  // theta_k / phi_k (:1157-1265); the theta prior lives in the theta slot
+  template <int k, bool IS_TH>
+  FDT_DEV void prop_angle(float n, float lu) {
+    constexpr int SL = IS_TH ? TH(k) : PH(k);
+    float tn = v<TH(k)>(), pn = v<PH(k)>(), prn = 0.0f, prold = 0.0f, xn;
+    if constexpr (IS_TH) { tn = xn = tn + n * width<SL>(); prn = theta_prior(tn); prold = prior<SL>(); }
+    else pn = xn = pn + n * width<SL>();
+    float st, ct, sp, cp;
+    __sincosf(tn, &st, &ct);
+    __sincosf(pn, &sp, &cp);
+    const float x0 = st * cp, x1 = st * sp, x2 = ct;
+    const SigC ca = aniso_coef(model, v<SL_D>(), v<SL_DSTD>(), v<SL_R>());
+    const float fk = v<FK(k)>(), S0 = v<SL_S0>(), tau = v<SL_TAU>();
+    reload_acq();
+    float sn[VPL], en[RIC ? VPL : 1];
+    float a = 0.0f;
+    by_kind(ca.kind, [&](auto SK) {
+#pragma unroll
+      for (int t = 0; t < VPL; ++t) {
+        float4 g = acq[rl + 16 * t];
+        float dp = fmaf(g.x, x0, fmaf(g.y, x1, g.z * x2));
+        sn[t] = sig<decltype(SK)::value>(ca, g.w * dp * dp, g.w);
+      }
+    });
+    Quad qd;
+#pragma unroll
+    for (int t = 0; t < VPL; ++t) {
+      if constexpr (RIC) {
+        en[t] = ell_of(r[t], S0 * fmaf(fk, sn[t] - s[k][t], M[t]), tau);
+        a += mask_last(t, en[t] - ell[t]);
+      } else qd.add(r[t], mask_last(t, sn[t] - s[k][t]));
+    }
+    a = RIC ? row_sum16(a) : qd.delta(S0 * fk);
+    if (dlik(a) + (prn - prold) < -lu) {
+      reload_acq();
+#pragma unroll
+      for (int t = 0; t < VPL; ++t) {
+        float dM = fk * (sn[t] - s[k][t]);
+        M[t] += dM;
+        if constexpr (RIC) ell[t] = en[t]; else r[t] -= S0 * dM;
+        s[k][t] = sn[t];
+        // b*(g.x)^2 of the accepted direction, recomputed rather than kept
+        // live through the row sum (the same expression as above)
+        float4 g = acq[rl + 16 * t];
+        float dp = fmaf(g.x, x0, fmaf(g.y, x1, g.z * x2));
+        bal[(k * VPL + t) * 64] = g.w * dp * dp;
+      }
+      like += (double)a;
+      if constexpr (IS_TH) accept<SL>(xn, prn);
+      else if (rl == SL) { sval = xn; scnt += 1; }
+    } else reject<SL>();
+  }
+  // f_k (:1267-1322)
+  template <int k>
+  FDT_DEV void prop_f(float n, float lu) {
+    constexpr int SL = FK(k);
+    const Fr q = fractions();
+    const float fn = q.fk[k] + n * width<SL>();
+    const float fsn = q.sum - q.fk[k] + fn;
+    if (fn <= 0.0f || fn >= 1.0f || fsn > 1.0f) { reject<SL>(); return; }
+    const float prn = (can_ard && lam[k]) ? fudge * flog(fn) : 0.0f;
+    const float df = fn - q.fk[k], S0 = v<SL_S0>(), tau = v<SL_TAU>();
+    float a = 0.0f, en[RIC ? VPL : 1];
+    Quad qd;
+#pragma unroll
+    for (int t = 0; t < VPL; ++t) {
+      if constexpr (RIC) {
+        en[t] = ell_of(r[t], S0 * fmaf(df, s[k][t] - iso[t], M[t]), tau);
+        a += mask_last(t, en[t] - ell[t]);
+      } else qd.add(r[t], mask_last(t, s[k][t] - iso[t]));
+    }
+    a = RIC ? row_sum16(a) : qd.delta(S0 * df);
+    if (dlik(a) + (prn - prior<SL>()) < -lu) {
+#pragma unroll
+      for (int t = 0; t < VPL; ++t) {
+        float dM = df * (s[k][t] - iso[t]);
+        M[t] += dM;
+        if constexpr (RIC) ell[t] = en[t]; else r[t] -= S0 * dM;
+      }
+      like += (double)a;
+      accept<SL>(fn, prn);
+    } else reject<SL>();
+  }
+
+  // proposal-width adaptation every upe jumps (:1341-1395), all slots of the
+  // row in parallel; slots of parameters that are not sampled keep counters
+  // 0/0 and are never read
+  FDT_DEV void update_proposals() {
+    float acc = (float)(scnt & 0xFFFF), rej = (float)(scnt >> 16);
+    sprop *= sqrtf((acc + 1.0f) / (rej + 1.0f));
+    sprop = fminf(sprop, MAXPROP_F);
+    scnt = 0;
+  }
+};
+
+// LDS of a chain workgroup: the acquisition table, b*(g.x_k)^2 per wave, the
+// b-shell index of each volume and the shells' b values, the rows' isotropic
+// shell tables
+struct ChainLds {
+  float4* acq;
+  float* ba;
+  int* sh;
+  float* shb;
+  float* iso;
+};
+
+// stage the acquisition of `a` (16*VPL volumes, zero padded) into LDS; the
+// caller synchronises the workgroup afterwards
+template <int VPL>
+FDT_DEV void stage_acq(const McmcArgs& a, const ChainLds& L) {
+  for (int i = threadIdx.x; i < 16 * VPL; i += blockDim.x) {
+    L.acq[i] = a.acq[i];
+    L.sh[i] = a.nshell ? a.shell[i] : 0;
+  }
+  if (threadIdx.x < 16) L.shb[threadIdx.x] = a.nshell ? a.shell_b[threadIdx.x] : 0.0f;
+}
+
+// The whole chain of voxel `vox` on this thread's 16-lane row: initialisation
+// (init_Fibres_Multifibres_kernel, CUDA/runmcmc_kernels.cu:193-505), nburn +
+// njumps sweeps with recording and proposal adaptation (runmcmc_kernel,
+// :507-1474, run twice by CUDA/runmcmc.cu:108-316).
+template <int K, int VPL, bool RIC>
+FDT_DEV void run_chain(const McmcArgs& a, int vox, const ChainLds& L) {
+  using C = Chain<K, VPL, RIC>;
+  const int lane = __lane_id();
+  const int rl = lane & 15;
+  const float4* sacq = L.acq;
+  C c;
+  c.rl = rl;
+  c.acq = sacq;
+  c.bal = L.ba + (threadIdx.x >> 6) * (K * VPL * 64) + lane;
+  c.sh = L.sh;
+  c.shb = L.shb;
+  c.isot = L.iso + (threadIdx.x >> 4) * 16;
+  c.nshell = a.nshell;
+  c.model = a.model;
+  c.incf0 = a.f0 != 0;
+  c.ardf0 = a.ardf0 != 0;
+  c.can_ard = a.no_ard == 0;
+  c.fudge = a.fudge;
+  c.halfN = 0.5f * (float)a.N;
+  c.Nf = (float)a.N;
+  c.Rmean = a.Rmean; c.Rstd = a.Rstd; c.Rfudge = a.Rfudge;
+  c.last_ok = (rl + 16 * (VPL - 1)) < a.N;
+#pragma unroll
+  for (int k = 0; k < K; ++k) c.lam[k] = (k == 0) ? (a.all_ard != 0) : (a.no_ard == 0);
+
+  // ---- data (with the Rician non-positive replacement, CUDA/xfibres_gpu.cu:350-367)
+  float y[VPL];
+  const float* yrow = a.data + (size_t)vox * a.N;
+#pragma unroll
+  for (int t = 0; t < VPL; ++t) {
+    int i = rl + 16 * t;
+    y[t] = i < a.N ? yrow[i] : 0.0f;
+  }
+  if (RIC) {
+    float mn = 3.0e38f, mx = -3.0e38f, mnp = 3.0e38f;
+#pragma unroll
+    for (int t = 0; t < VPL; ++t)
+      if (rl + 16 * t < a.N) { mn = fminf(mn, y[t]); mx = fmaxf(mx, y[t]); if (y[t] > 0.0f) mnp = fminf(mnp, y[t]); }
+    mn = row_min16(mn); mx = row_max16(mx); mnp = row_min16(mnp);
+    bool san = (mn <= 0.0f && mx > 0.0f);
+#pragma unroll
+    for (int t = 0; t < VPL; ++t) {
+      if (san && y[t] <= 0.0f) y[t] = mnp;
+      if (rl + 16 * t >= a.N) y[t] = 1.0f;
+    }
+  }
+
+  // ---- chain initialisation (init_Fibres_Multifibres_kernel, :265-504):
+  // every lane loads the parameter of its slot and evaluates its prior
+  const float* P = a.params + (size_t)vox * a.np;
+  const int add = a.model >= 2 ? 1 : 0;
+  float d = P[1];
+  if (d < 0.0f || d > UPPERDIFF_F) d = 2e-3f;
+  float dstd = 0.0f, Rr = 0.0f;
+  if (a.model >= 2) {
+    dstd = P[2];
+    float up = a.model == 3 ? 0.004f : 0.01f;
+    if (dstd < 0.0f || dstd > up) dstd = d / 10.0f;
+    if (a.model == 3) Rr = a.Rmean;
+  }
+  float val = 0.0f, pr = 0.0f, wd = 0.2f;  // f0 and the fibre slots start at width 0.2 (:497-503)
+  if (rl == SL_F0) {
+    val = c.incf0 ? P[a.np - 1] : 0.0f;
+    if (c.incf0 && c.ardf0 && val > 0.0f && val < 1.0f) pr = flog(val);
+  } else if (rl == SL_TAU) {
+    val = (RIC && a.tau) ? a.tau[vox] : 0.0f;
+    wd = val / 2.0f;
+  } else if (rl == SL_D) {
+    val = d;
+    if (a.model == 3) pr = (1.0f - 3.0f) * flog(d) + 4000.0f * d;
+    wd = d / 10.0f;
+  } else if (rl == SL_DSTD) {
+    val = dstd;
+    float up = a.model == 3 ? 0.004f : 0.01f;
+    if (a.model >= 2 && dstd > 0.0f && dstd <= up) pr = flog(dstd);
+    wd = dstd / 10.0f;
+  } else if (rl == SL_R) {
+    val = Rr;
+    float q;
+    if (a.model == 3 && c.r_prior(Rr, d, q)) pr = q;
+    wd = Rr / 10.0f;
+  } else if (rl == SL_S0) {
+    val = P[0];
+    wd = val / 10.0f;
+  } else if (rl < SL_FIB + 3 * K) {
+    int k = (rl - SL_FIB) / 3, w = (rl - SL_FIB) % 3;
+    if (w == 0) { val = P[3 + 3 * k + add]; pr = C::theta_prior(val); }
+    else if (w == 1) val = P[4 + 3 * k + add];
+    else {
+      val = P[2 + 3 * k + add];
+      bool lk = (k == 0) ? (a.all_ard != 0) : (a.no_ard == 0);
+      pr = (val <= 0.0f || val >= 1.0f) ? 0.0f : a.fudge * (lk ? flog(val) : 0.0f);
+    }
+  }
+  c.sval = val;
+  c.sprior = pr;
+  c.sprop = wd;
+  c.scnt = 0;
+  {
+    SigC ca = aniso_coef(a.model, d, dstd, Rr), ci = iso_coef(a.model, d, dstd);
+    sfor<K>([&](auto kc) {
+      constexpr int k = kc;
+      float st, ct, sp, cp;
+      __sincosf(c.template v<C::TH(k)>(), &st, &ct);
+      __sincosf(c.template v<C::PH(k)>(), &sp, &cp);
+      float x0 = st * cp, x1 = st * sp, x2 = ct;
+#pragma unroll
+      for (int t = 0; t < VPL; ++t) {
+        float4 g = sacq[rl + 16 * t];
+        float dp = fmaf(g.x, x0, fmaf(g.y, x1, g.z * x2));
+        c.bal[(k * VPL + t) * 64] = g.w * dp * dp;
+      }
+    });
+    by_kind(ca.kind, [&](auto SK) {
+#pragma unroll
+      for (int t = 0; t < VPL; ++t)
+#pragma unroll
+        for (int k = 0; k < K; ++k) c.s[k][t] = sig<decltype(SK)::value>(ca, c.ba(k, t), sacq[rl + 16 * t].w);
+    });
+    by_kind(ci.kind, [&](auto SK) {
+#pragma unroll
+      for (int t = 0; t < VPL; ++t) { float b = sacq[rl + 16 * t].w; c.iso[t] = sig<decltype(SK)::value>(ci, b, b); }
+    });
+  }
+  if constexpr (RIC) {
+#pragma unroll
+    for (int t = 0; t < VPL; ++t) c.r[t] = y[t];
+  }
+  c.rebuild(y);
+
+  // record destination of the lane's slot (:1326-1339)
+  float* rec_base = nullptr;
+  size_t rec_stride = (size_t)a.nvox;  // per sample
+  if (rl == SL_F0) rec_base = a.f0s;
+  else if (rl == SL_TAU) rec_base = a.taus;
+  else if (rl == SL_D) rec_base = a.d;
+  else if (rl == SL_DSTD) rec_base = a.dstd;
+  else if (rl == SL_R) rec_base = a.R;
+  else if (rl == SL_S0) rec_base = a.S0;
+  else if (rl < SL_FIB + 3 * K) {
+    int k = (rl - SL_FIB) / 3, w = (rl - SL_FIB) % 3;
+    float* arr = w == 0 ? a.th : (w == 1 ? a.ph : a.f);
+    if (arr) rec_base = arr + (size_t)k * a.nsamples * a.nvox;
+  }
+  if (rec_base) rec_base += vox;
+
+  const uint64_t gvox = a.vox_offset + (uint64_t)vox;
+  const int total = a.nburn + a.njumps;
+  for (int j = 0; j < total; ++j) {
+    float nl, lul;
+    draw(a.seed, gvox, (uint32_t)j, (uint32_t)rl, nl, lul);
+    if (c.incf0) c.prop_f0(row_bcast<SL_F0>(nl), row_bcast<SL_F0>(lul));
+    if (RIC) c.prop_tau(row_bcast<SL_TAU>(nl), row_bcast<SL_TAU>(lul));
+    c.template prop_global<SL_D>(row_bcast<SL_D>(nl), row_bcast<SL_D>(lul));
+    if (a.model >= 2) {
+      c.template prop_global<SL_DSTD>(row_bcast<SL_DSTD>(nl), row_bcast<SL_DSTD>(lul));
+      if (a.model == 3) c.template prop_global<SL_R>(row_bcast<SL_R>(nl), row_bcast<SL_R>(lul));
+    }
+    c.prop_S0(row_bcast<SL_S0>(nl), row_bcast<SL_S0>(lul));
+    sfor<K>([&](auto kc) {
+      constexpr int k = kc;
+      c.template prop_angle<k, true>(row_bcast<C::TH(k)>(nl), row_bcast<C::TH(k)>(lul));
+      c.template prop_angle<k, false>(row_bcast<C::PH(k)>(nl), row_bcast<C::PH(k)>(lul));
+      c.template prop_f<k>(row_bcast<C::FK(k)>(nl), row_bcast<C::FK(k)>(lul));
+    });
+
+    // record: the lane of each sampled slot writes its value
+    if (j >= a.nburn) {
+      int rec = j - a.nburn + 1;
+      if (a.se > 0 && rec % a.se == 0) {
+        int smp = rec / a.se - 1;
+        if (smp < a.nsamples && rec_base) rec_base[(size_t)smp * rec_stride] = c.sval;
+      }
+    }
+    if ((j + 1) % a.upe == 0) c.update_proposals();
+  }
+  if (a.energy) {
+    float tau = c.template v<SL_TAU>();
+    if (rl == 0) a.energy[vox] = RIC ? (float)(-(double)a.N * (double)flog(tau) - c.like) : (float)c.like;
+  }
+}
+
+
+}  // namespace fdt

@@ -65,6 +65,9 @@ struct McmcArgs {
   const float* params;  // [nvox][np] LM output
   const float* tau;     // [nvox] or null
   const float4* acq;    // [Npad] (gx, gy, gz, b), zero padded
+  const int* shell;     // [Npad] b-shell of each volume (nshell > 0)
+  const float* shell_b; // [16] b value of each shell
+  int nshell;           // distinct b values if <= 16, else 0
   int N, Npad, nvox, np;
   int model, f0, rician, ardf0, no_ard, all_ard;
   float fudge, Rmean, Rstd, Rfudge;
@@ -73,6 +76,21 @@ struct McmcArgs {
   // raw samples: th/ph/f [K][ns][nvox]; scalars [ns][nvox]
   float *th, *ph, *f, *S0, *d, *dstd, *R, *f0s, *taus;
   float* energy;        // [nvox] final SSR (Gaussian) / likelihood energy, optional
+};
+
+// Batched MCMC over a queue of volumes (fdt_queue.hip): work items are runs
+// of rows_per_wg consecutive voxels of one job; job j owns items
+// [first_item[j], first_item[j+1]).
+constexpr int QUEUE_MAX_VPL = 18;   // tiles compiled into the queue kernel: VPL 3, 5, 10, 18
+struct QueueJob {
+  McmcArgs a;
+  int K, vpl;
+};
+struct QueueArgs {
+  const QueueJob* jobs;
+  const int* first_item;  // [njobs + 1]
+  int njobs;
+  int* counter;           // work-queue counter, zero at launch
 };
 
 struct FinishArgs {

@@ -177,6 +177,35 @@ int fdt_get_timing(fdt_ctx* c, fdt_timing* t);
  * the reference (its MCMC block shape is fixed, CUDA/runmcmc.cu:108-316). */
 int fdt_set_mcmc_tile(fdt_ctx* c, int vpl);
 
+/* ---------------------------------------------------------------- batches
+ * A queue of volumes with their own acquisitions, models and fibre counts,
+ * run together (the reference runs one volume per process and sub-part,
+ * CUDA/xfibres_gpu.cu:29-146, and has no batching): the LM stages of every
+ * volume run concurrently, then one persistent chain launch covers every
+ * volume whose tile the queue kernel carries (Gaussian noise, N <= 288),
+ * then the finishing passes.  The first fdt_run_queue runs eagerly and
+ * records the launches into a hipGraph that later runs replay. */
+typedef struct fdt_queue fdt_queue;
+fdt_queue* fdt_queue_create(int device);
+void fdt_queue_destroy(fdt_queue* q);
+const char* fdt_queue_last_error(const fdt_queue* q);
+/* Add a volume: its options, acquisition and data (copied to the device).
+ * Returns the job index (>= 0) or a negative FDT_E* code. */
+int fdt_enqueue_batch(fdt_queue* q, const fdt_opts* o, int N, const float* bvals, const float* bvecs,
+                      const float* data, int layout, int nvox, uint64_t vox_offset);
+int fdt_run_queue(fdt_queue* q);
+int fdt_queue_num_jobs(const fdt_queue* q);
+int fdt_queue_get_outputs(fdt_queue* q, int job, const fdt_outputs* out);
+int fdt_queue_get_params(fdt_queue* q, int job, float* params, float* tau);
+/* Voxels per workgroup of the queue kernel (4, 8, 16; 0 = keep) and the
+ * volumes-per-lane tile of one job (job < 0: none; vpl 0 = automatic). */
+int fdt_queue_set_tiles(fdt_queue* q, int rows_per_wg, int job, int vpl);
+/* Time the queue kernel on the first sample_vox voxels of every volume for
+ * `jumps` sweeps over the tile choices and keep the fastest. */
+int fdt_queue_autotune(fdt_queue* q, int jumps, int sample_vox, float* ms_best);
+/* Duration of the last fdt_run_queue (ms) and the tiles in use. */
+int fdt_queue_timing(fdt_queue* q, float* ms_last, int* rows_per_wg, int* vpl_of_job, int njobs);
+
 #ifdef __cplusplus
 }
 #endif
