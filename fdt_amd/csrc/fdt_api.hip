@@ -853,7 +853,13 @@ static int queue_issue(fdt_queue* q, bool upload) {
   if (upload) {
     q->h_jobs.clear();
     q->h_first.assign(1, 0);
-    for (fdt_ctx* c : q->jobs) {
+    // items of the costliest volumes first (cost per voxel ~ N * (K + 1)), so
+    // that the launch ends on short items
+    std::vector<fdt_ctx*> order(q->jobs.begin(), q->jobs.end());
+    std::stable_sort(order.begin(), order.end(), [](const fdt_ctx* x, const fdt_ctx* y) {
+      return (long)x->N * (x->o.nfibres + 1) > (long)y->N * (y->o.nfibres + 1);
+    });
+    for (fdt_ctx* c : order) {
       if (!in_queue_kernel(c)) continue;
       QueueJob j{};
       mcmc_launch_fn fn = nullptr;
@@ -1029,7 +1035,11 @@ int fdt_queue_autotune(fdt_queue* q, int jumps, int sample_vox, float* ms_best) 
   auto time_config = [&](float* ms) -> int {
     std::vector<QueueJob> tj;
     std::vector<int> first(1, 0);
-    for (fdt_ctx* c : q->jobs) {
+    std::vector<fdt_ctx*> order(q->jobs.begin(), q->jobs.end());
+    std::stable_sort(order.begin(), order.end(), [](const fdt_ctx* x, const fdt_ctx* y) {
+      return (long)x->N * (x->o.nfibres + 1) > (long)y->N * (y->o.nfibres + 1);
+    });
+    for (fdt_ctx* c : order) {
       if (!in_queue_kernel(c)) continue;
       QueueJob j{};
       mcmc_launch_fn fn = nullptr;
