@@ -88,6 +88,92 @@ def cpu_baseline(v, opts, seconds=15.0, threads=None):
                        f"({t_used:.1f} s of CPU work); value_1core: {done1} voxels on 1 thread ({t1:.1f} s)")
 
 
+SURROGATE_WEIGHTS = os.path.join(HERE, "fdt_amd", "weights", "surrogate_c1_model2_nf3.npz")
+SG_DEFAULTS = dict(nburn_short=200, chi2_max=3.0, ang_max_deg=30.0)
+
+
+def surrogate_eval(v, opts, s, steps, plain_out=None):
+    """time the learned-start path on the resident volume and compare dyad errors with the plain path"""
+    import fdt_amd as F
+    from fdt_amd import surrogate as SG
+    nvox = v["data"].shape[0]
+    x = F.Xfibres(**opts)
+    x.set_acquisition(v["bvals"], v["bvecs"])
+    x.load_data(v["data"])
+    x.run_surrogate(s, **SG_DEFAULTS)
+    t = time.perf_counter()
+    for _ in range(steps):
+        nfb = x.run_surrogate(s, **SG_DEFAULTS)
+    ms = (time.perf_counter() - t) / steps * 1e3
+    out = x.outputs(raw_scalars=False)
+    x.close()
+    e = SG.dyad_errors(out["dyads"], out["mean_f"], v["dirs"], v["f"])
+    res = {"value": nvox / (ms / 1e3), "unit": "voxels/s", "ms_per_step": ms, "fallback_frac": nfb / nvox,
+           "dyad_err_median_deg": float(np.nanmedian(e)), "unmatched_fibres": float(np.mean(np.isnan(e))),
+           **SG_DEFAULTS}
+    if plain_out is not None:
+        ep = SG.dyad_errors(plain_out["dyads"], plain_out["mean_f"], v["dirs"], v["f"])
+        res["plain_dyad_err_median_deg"] = float(np.nanmedian(ep))
+        res["plain_unmatched_fibres"] = float(np.mean(np.isnan(ep)))
+    return res
+
+
+def bench_c3(args):
+    """Learned start on the multi-shell volume: train the network from scratch
+    on the engine's own labels (plain LM + MCMC on seeded voxels of the same
+    acquisition, disjoint from the evaluation voxels), then time the learned
+    path (burn-in 200, plain fallback) against the plain path and compare
+    median dyad errors against the synthetic truth."""
+    import fdt_amd as F
+    from fdt_amd import synth, surrogate as SG
+    opts = dict(nfibres=3, model=2, nburn=1000, njumps=1250, sampleevery=25)
+    t0 = time.perf_counter()
+    tr = synth.voxels("c1", args.train_voxels, sub=1, with_truth=False)
+    t_syn = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    T = SG.label(tr["bvals"], tr["bvecs"], tr["data"], **opts)
+    t_lab = time.perf_counter() - t0
+    s = F.Surrogate(tr["bvals"].size, seed=11)
+    t0 = time.perf_counter()
+    losses = SG.train(s, tr["data"], tr["bvals"], T, epochs=args.epochs, batch=1024, lr=1e-3, lr_decay=0.8)
+    t_tr = time.perf_counter() - t0
+    del tr, T
+    if args.save_weights:
+        os.makedirs(os.path.dirname(os.path.abspath(args.save_weights)), exist_ok=True)
+        np.savez_compressed(args.save_weights, weights=s.weights(), n_in=s.n_in, train_voxels=args.train_voxels,
+                            epochs=args.epochs, loss=np.array(losses))
+    nvox = args.nvox or synth.C1_NVOX
+    v = synth.voxels("c1", nvox)
+    x = F.Xfibres(**opts)
+    x.set_acquisition(v["bvals"], v["bvecs"])
+    x.load_data(v["data"])
+    x.run(0)
+    x.sync()
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        x.run(0)
+        x.sync()
+    plain_ms = (time.perf_counter() - t) / args.steps * 1e3
+    plain_out = x.outputs(raw_scalars=False)
+    x.close()
+    sg = surrogate_eval(v, opts, s, args.steps, plain_out)
+    s.close()
+    line = {
+        "metric": METRIC, "value": sg["value"], "unit": "voxels/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": 1, "ms_per_step": sg["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "fp32 chains, bf16 network",
+        "data": "synthetic (seeded Philox), no published dataset",
+        "config": {"workload": f"learned start on multi-shell model 2, nf=3, N=288, {nvox} voxels, burn-in 200 "
+                               "(fallback: plain LM + burn-in 1000), nj=1250 se=25",
+                   "plain_voxels_per_s": nvox / (plain_ms / 1e3), "speedup_vs_plain": plain_ms / sg["ms_per_step"],
+                   "surrogate": sg,
+                   "training": {"voxels": args.train_voxels, "epochs": args.epochs, "final_loss": losses[-1],
+                                "s_synth": t_syn, "s_label": t_lab, "s_train": t_tr,
+                                "network": "MLP 288-512-512-256-28, GELU, BF16 MFMA, FP32 accumulation, Adam"}},
+    }
+    print(json.dumps(line), flush=True)
+
+
 C2_MIX = [("c2_33", 1, 1), ("c2_69", 2, 2), ("c2_145", 2, 3), ("c2_288", 3, 3)]
 
 
@@ -156,8 +242,12 @@ def main():
     ap.add_argument("--nvox", type=int, default=0, help="override the voxel count (default: the full mask)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--workload", default="c1", choices=["c1", "c2"],
-                    help="c1: the multi-shell 3-fibre volume (default); c2: the mixed-acquisition queue")
+    ap.add_argument("--workload", default="c1", choices=["c1", "c2", "c3"],
+                    help="c1: the multi-shell 3-fibre volume (default); c2: the mixed-acquisition queue; "
+                         "c3: train the learned start and compare it with the plain path")
+    ap.add_argument("--train-voxels", type=int, default=2097152)
+    ap.add_argument("--epochs", type=int, default=10)
+    ap.add_argument("--save-weights", default="")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: --gpus {args.gpus} is not supported: this engine runs on one GPU per process "
@@ -167,12 +257,15 @@ def main():
     if args.workload == "c2":
         bench_c2(args)
         return
+    if args.workload == "c3":
+        bench_c3(args)
+        return
 
     import fdt_amd as F
     from fdt_amd import synth
 
     nvox = args.nvox or synth.C1_NVOX
-    v = synth.voxels("c1", nvox, with_truth=False)
+    v = synth.voxels("c1", nvox)
     N = v["bvals"].size
     opts = dict(nfibres=3, model=2, nburn=1000, njumps=1250, sampleevery=25)
     x = F.Xfibres(**opts)
@@ -195,6 +288,7 @@ def main():
     ms_step = elapsed / args.steps * 1e3
     value = nvox / (ms_step / 1e3)
     tm = x.timing()
+    plain_out = x.outputs(raw_scalars=False)
     x.close()
 
     # end to end once more, from host buffers: upload, run, download of every output
@@ -230,6 +324,17 @@ def main():
                                 "is the reachable issue ceiling"},
         "end_to_end_voxels_per_s": e2e,
     }
+    if os.path.exists(SURROGATE_WEIGHTS):
+        import fdt_amd as F2
+        wz = np.load(SURROGATE_WEIGHTS)
+        s = F2.Surrogate(N, seed=0)
+        s.load(wz["weights"])
+        sg = surrogate_eval(v, opts, s, args.steps, plain_out)
+        s.close()
+        sg["speedup_vs_plain"] = sg["value"] / value
+        sg["weights"] = (f"fdt_amd/weights/{os.path.basename(SURROGATE_WEIGHTS)}: trained by bench.py --workload c3 "
+                         f"on {int(wz['train_voxels'])} voxels, {int(wz['epochs'])} epochs")
+        line["surrogate"] = sg
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(v, opts, seconds=args.cpu_seconds)
     print(json.dumps(line), flush=True)

@@ -26,7 +26,10 @@ EXPORTS = ["fdt_default_opts", "fdt_create", "fdt_destroy", "fdt_last_error", "f
            "fdt_get_timing", "fdt_set_mcmc_tile",
            "fdt_queue_create", "fdt_queue_destroy", "fdt_queue_last_error", "fdt_enqueue_batch", "fdt_run_queue",
            "fdt_queue_num_jobs", "fdt_queue_get_outputs", "fdt_queue_get_params", "fdt_queue_set_tiles",
-           "fdt_queue_autotune", "fdt_queue_timing"]
+           "fdt_queue_autotune", "fdt_queue_timing", "fdt_get_widths",
+           "fdt_surrogate_create", "fdt_surrogate_destroy", "fdt_surrogate_last_error", "fdt_surrogate_num_weights",
+           "fdt_surrogate_load", "fdt_surrogate_save", "fdt_surrogate_set_training_data", "fdt_surrogate_gradients",
+           "fdt_surrogate_train_step", "fdt_surrogate_forward", "fdt_surrogate_infer", "fdt_run_surrogate", "fdt_get_fallback_flags"]
 
 
 class Opts(C.Structure):
@@ -105,6 +108,24 @@ def lib():
         L.fdt_queue_set_tiles.argtypes = [vp, C.c_int, C.c_int, C.c_int]
         L.fdt_queue_autotune.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_float)]
         L.fdt_queue_timing.argtypes = [vp, C.POINTER(C.c_float), ip, ip, C.c_int]
+        L.fdt_get_widths.argtypes = [vp, _FP]
+        L.fdt_surrogate_create.argtypes = [C.c_int, C.c_int, C.c_uint64]
+        L.fdt_surrogate_create.restype = vp
+        L.fdt_surrogate_destroy.argtypes = [vp]
+        L.fdt_surrogate_destroy.restype = None
+        L.fdt_surrogate_last_error.argtypes = [vp]
+        L.fdt_surrogate_last_error.restype = C.c_char_p
+        L.fdt_surrogate_num_weights.argtypes = [vp]
+        L.fdt_surrogate_num_weights.restype = C.c_size_t
+        L.fdt_surrogate_load.argtypes = [vp, _FP, C.c_size_t]
+        L.fdt_surrogate_save.argtypes = [vp, _FP, C.c_size_t]
+        L.fdt_surrogate_set_training_data.argtypes = [vp, _FP, _FP, _FP, C.c_int]
+        L.fdt_surrogate_gradients.argtypes = [vp, C.c_int, C.c_int, _FP, _FP]
+        L.fdt_surrogate_train_step.argtypes = [vp, C.c_int, C.c_int, C.c_float, _FP]
+        L.fdt_surrogate_forward.argtypes = [vp, _FP, _FP, C.c_int, _FP]
+        L.fdt_surrogate_infer.argtypes = [vp, vp]
+        L.fdt_run_surrogate.argtypes = [vp, vp, C.c_uint64, C.c_int, C.c_float, C.c_float, ip]
+        L.fdt_get_fallback_flags.argtypes = [vp, C.POINTER(C.c_uint8)]
         _lib = L
     return _lib
 
@@ -227,6 +248,27 @@ class Xfibres:
     def set_mcmc_tile(self, vpl):
         self._chk(self.L.fdt_set_mcmc_tile(self.h, vpl))
 
+    def widths(self):
+        w = np.zeros((self.nvox, 16), np.float32)
+        self._chk(self.L.fdt_get_widths(self.h, w.ctypes.data_as(_FP)))
+        return w
+
+    def run_surrogate(self, surrogate, vox_offset=0, nburn_short=200, chi2_max=2.0, ang_max_deg=20.0):
+        """learned start + short burn-in, plain fallback; returns the number of rerun voxels"""
+        n = C.c_int()
+        rc = self.L.fdt_run_surrogate(self.h, surrogate.h, vox_offset, nburn_short, chi2_max, ang_max_deg, C.byref(n))
+        if rc < 0:
+            raise FdtError(self.L.fdt_last_error(self.h).decode())
+        return n.value
+
+    def fallback_flags(self):
+        f = np.zeros(self.nvox, np.uint8)
+        self._chk(self.L.fdt_get_fallback_flags(self.h, f.ctypes.data_as(C.POINTER(C.c_uint8))))
+        return f
+
+    def surrogate_start(self, surrogate):
+        surrogate._chk(self.L.fdt_surrogate_infer(surrogate.h, self.h))
+
     def outputs(self, raw_scalars=True):
         arr, out = _alloc_outputs(self.o, self.nvox, raw_scalars)
         self._chk(self.L.fdt_get_outputs(self.h, C.byref(out)))
@@ -263,6 +305,84 @@ def _alloc_outputs(o, nv, raw_scalars=True):
     for n in _OUT_FIELDS:
         setattr(out, n, arr[n].ctypes.data_as(_FP) if n in arr else _FP())
     return arr, out
+
+
+SG_OUT = 28
+# proposal slots whose log widths the network predicts (f0, d, d_std, R, S0, th/ph/f of three fibres)
+SG_WIDTH_SLOTS = [0, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14]
+
+
+class Surrogate:
+    """The learned start (fdt_surrogate_* of the C ABI): MLP N -> 512 -> 512 ->
+    256 -> 28 with BF16 MFMA kernels, trained on the GPU."""
+
+    def __init__(self, n_in: int, seed: int = 1, device: int = 0):
+        self.L = lib()
+        self.h = self.L.fdt_surrogate_create(device, n_in, seed)
+        if not self.h:
+            raise FdtError("fdt_surrogate_create failed")
+        self.n_in = n_in
+        self.nw = self.L.fdt_surrogate_num_weights(self.h)
+
+    def close(self):
+        if self.h:
+            self.L.fdt_surrogate_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def _chk(self, rc):
+        if rc < 0:
+            raise FdtError(self.L.fdt_surrogate_last_error(self.h).decode())
+        return rc
+
+    def weights(self):
+        w = np.zeros(self.nw, np.float32)
+        self._chk(self.L.fdt_surrogate_save(self.h, w.ctypes.data_as(_FP), self.nw))
+        return w
+
+    def load(self, w):
+        w, wp = _f32(w)
+        self._chk(self.L.fdt_surrogate_load(self.h, wp, w.size))
+
+    def layers(self, w=None):
+        """(W, b) per layer from a flat weight vector (W is in x out)"""
+        w = self.weights() if w is None else w
+        dims = [self.n_in, 512, 512, 256, SG_OUT]
+        out, o = [], 0
+        for l in range(4):
+            W = w[o:o + dims[l] * dims[l + 1]].reshape(dims[l], dims[l + 1])
+            o += dims[l] * dims[l + 1]
+            b = w[o:o + dims[l + 1]]
+            o += dims[l + 1]
+            out.append((W, b))
+        return out
+
+    def set_training_data(self, signals, bvals, targets):
+        x, xp = _f32(signals)
+        b, bp = _f32(bvals)
+        t, tp = _f32(targets)
+        self._chk(self.L.fdt_surrogate_set_training_data(self.h, xp, bp, tp, x.shape[0]))
+        self.ntrain = x.shape[0]
+
+    def gradients(self, first, batch):
+        g = np.zeros(self.nw, np.float32)
+        loss = C.c_float()
+        self._chk(self.L.fdt_surrogate_gradients(self.h, first, batch, g.ctypes.data_as(_FP), C.byref(loss)))
+        return g, loss.value
+
+    def train_step(self, first, batch, lr):
+        loss = C.c_float()
+        self._chk(self.L.fdt_surrogate_train_step(self.h, first, batch, lr, C.byref(loss)))
+        return loss.value
+
+    def forward(self, signals, bvals):
+        x, xp = _f32(signals)
+        b, bp = _f32(bvals)
+        out = np.zeros((x.shape[0], SG_OUT), np.float32)
+        self._chk(self.L.fdt_surrogate_forward(self.h, xp, bp, x.shape[0], out.ctypes.data_as(_FP)))
+        return out
 
 
 class Queue:

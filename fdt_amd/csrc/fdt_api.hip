@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <random>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -17,6 +18,7 @@
 
 #include "../../include/fdt_xfibres.h"
 #include "fdt_kernels.h"
+#include "fdt_mlp.h"
 
 using namespace fdt;
 
@@ -33,6 +35,16 @@ FDT_LM_DECL(3, 1)
 hipError_t fdt_launch_f0_flag(const F0FlagArgs*, hipStream_t);
 hipError_t fdt_launch_zero_ints(int*, int, hipStream_t);
 hipError_t fdt_launch_mcmc_queue(const QueueArgs*, int rows_per_wg, int max_items, hipStream_t);
+hipError_t fdt_launch_gemm_bf16(const GemmArgs*, hipStream_t);
+hipError_t fdt_launch_mlp_ew(int op, const void* a, const void* b, void* c, size_t n, int M, int N, hipStream_t);
+hipError_t fdt_launch_adam(float* w, float* m, float* v, const float* g, uint16_t* wb, size_t n, float lr, float b1,
+                           float b2, float eps, int step, hipStream_t);
+hipError_t fdt_launch_normalize(const float* data, int N, int nvox, const float* bvals, float b0_max, uint16_t* X,
+                                float* S0, hipStream_t);
+hipError_t fdt_launch_sg_loss(const float* O, const float* T, int M, uint16_t* dO, float* loss_part, float w_lw,
+                              hipStream_t);
+hipError_t fdt_launch_sg_decode(const SgDecodeArgs*, hipStream_t);
+hipError_t fdt_launch_sg_fallback(const SgFallbackArgs*, hipStream_t);
 hipError_t fdt_launch_finish(const FinishArgs*, hipStream_t);
 hipError_t fdt_launch_transpose(const float*, float*, int, int, hipStream_t);
 #define FDT_VPL_DECL(V) hipError_t fdt_launch_mcmc_vpl##V(const McmcArgs*, int, int, hipStream_t);
@@ -159,6 +171,8 @@ struct fdt_ctx {
   DevBuf<float> d_data, d_stage, d_params, d_tau, d_energy;
   DevBuf<int> d_iters, d_queue, d_list;
   DevBuf<float> d_s1;             // finalised stage-1 vectors of the model >= 2 pipeline
+  DevBuf<float> d_widths_in, d_widths_out;  // proposal widths per voxel and slot [nvox][16]
+  bool use_widths_in = false;
   DevBuf<unsigned char> d_refit;  // voxels refitted without f0
   bool have_params = false, have_samples = false, have_finish = false;
   // raw and sorted samples
@@ -249,6 +263,8 @@ void fdt_destroy(fdt_ctx* c) {
   c->d_list.release();
   c->d_s1.release();
   c->d_refit.release();
+  c->d_widths_in.release();
+  c->d_widths_out.release();
   for (auto& e : c->ev) if (e) (void)hipEventDestroy(e);
   if (c->ev_done) (void)hipEventDestroy(c->ev_done);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -625,6 +641,10 @@ static int mcmc_prepare(fdt_ctx* c, uint64_t vox_offset, McmcArgs* out, mcmc_lau
   a.f0s = o.f0 ? c->s_f0.p : nullptr;
   a.taus = o.rician ? c->s_tau.p : nullptr;
   a.energy = c->d_energy.p;
+  a.vstride = c->nvox;
+  HIPCHK(c, c->d_widths_out.ensure(nv * 16));
+  a.widths_out = c->d_widths_out.p;
+  a.widths_in = c->use_widths_in ? c->d_widths_in.p : nullptr;
   *out = a;
   *fn_out = fn;
   return FDT_OK;
@@ -1099,6 +1119,443 @@ int fdt_queue_autotune(fdt_queue* q, int jumps, int sample_vox, float* ms_best) 
   if (ms_best) *ms_best = best;
   queue_invalidate(q);
   q->ran = false;  // the timing runs overwrote sample 0 of each volume
+  return FDT_OK;
+}
+
+}  // extern "C"
+
+// -------------------------------------------------------------- surrogate
+// The learned initialiser: an MLP N -> 512 -> 512 -> 256 -> 28 (GELU between
+// layers, BF16 weights and activations, FP32 accumulation, FP32 master
+// weights and Adam moments).  Flat weight layout: W1 [N][512], b1, W2
+// [512][512], b2, W3 [512][256], b3, W4 [256][28], b4 (W[l] is in x out).
+
+struct fdt_surrogate {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int n_in = 0;
+  int dims[SG_LAYERS + 1] = {};
+  size_t woff[SG_LAYERS] = {}, boff[SG_LAYERS] = {}, nw = 0;
+  DevBuf<float> w, m, v, g;
+  DevBuf<uint16_t> wb;
+  int step = 0;
+  DevBuf<uint16_t> X;  // normalised training signals [n][n_in]
+  DevBuf<float> T;     // targets [n][SG_OUT]
+  DevBuf<float> bvals;
+  int ntrain = 0;
+  int cap = 0;  // batch capacity of the activation buffers
+  DevBuf<float> Z[SG_LAYERS];
+  DevBuf<uint16_t> A[SG_LAYERS];
+  DevBuf<float> dA;
+  DevBuf<uint16_t> dZ, Xb;
+  DevBuf<float> loss_part, S0;
+  std::string err;
+};
+
+static int serr(fdt_surrogate* s, int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  if (s) s->err = buf;
+  return code;
+}
+#define SHIP(s, expr)                                                                              \
+  do {                                                                                             \
+    hipError_t e_ = (expr);                                                                        \
+    if (e_ != hipSuccess) return serr(s, FDT_EHIP, "%s: %s", #expr, hipGetErrorString(e_));        \
+  } while (0)
+
+static int sg_reserve(fdt_surrogate* s, int b) {
+  if (b <= s->cap) return FDT_OK;
+  for (int l = 0; l < SG_LAYERS; ++l) {
+    SHIP(s, s->Z[l].ensure((size_t)b * s->dims[l + 1]));
+    SHIP(s, s->A[l].ensure((size_t)b * s->dims[l + 1]));
+  }
+  SHIP(s, s->dA.ensure((size_t)b * 512));
+  SHIP(s, s->dZ.ensure((size_t)b * 512));
+  SHIP(s, s->Xb.ensure((size_t)b * s->n_in));
+  SHIP(s, s->S0.ensure((size_t)b));
+  SHIP(s, s->loss_part.ensure((size_t)(b + 255) / 256));
+  s->cap = b;
+  return FDT_OK;
+}
+
+static GemmArgs gemm(int M, int N, int K, const uint16_t* A, long sam, long sak, const uint16_t* B, long sbk, long sbn) {
+  GemmArgs g{};
+  g.M = M; g.N = N; g.K = K;
+  g.A = A; g.sam = sam; g.sak = sak;
+  g.B = B; g.sbk = sbk; g.sbn = sbn;
+  g.ldc = N;
+  return g;
+}
+
+// forward of b rows of X (bf16 [b][n_in]); keeps Z (pre-activations) when
+// `keep`, the output in Z[3]
+static int sg_forward(fdt_surrogate* s, const uint16_t* X, int b, bool keep) {
+  const uint16_t* in = X;
+  for (int l = 0; l < SG_LAYERS; ++l) {
+    const int K = s->dims[l], N = s->dims[l + 1];
+    GemmArgs g = gemm(b, N, K, in, K, 1, s->wb.p + s->woff[l], N, 1);
+    g.bias = s->w.p + s->boff[l];
+    const bool last = l == SG_LAYERS - 1;
+    g.C = (keep || last) ? s->Z[l].p : nullptr;
+    g.Cb = last ? nullptr : s->A[l].p;
+    g.act = last ? 0 : 1;
+    SHIP(s, fdt_launch_gemm_bf16(&g, s->stream));
+    in = s->A[l].p;
+  }
+  return FDT_OK;
+}
+
+// gradients of the mean loss over b rows into s->g; loss sum into *loss (host)
+static int sg_backward(fdt_surrogate* s, const uint16_t* X, const float* T, int b, float* loss) {
+  const int nb = (b + 255) / 256;
+  SHIP(s, fdt_launch_sg_loss(s->Z[SG_LAYERS - 1].p, T, b, s->dZ.p, s->loss_part.p, 0.1f, s->stream));
+  for (int l = SG_LAYERS - 1; l >= 0; --l) {
+    const int K = s->dims[l], N = s->dims[l + 1];
+    const uint16_t* in = l ? s->A[l - 1].p : X;
+    // dW = in^T . dZ  (K x N, reduced over the batch)
+    GemmArgs gw = gemm(K, N, b, in, 1, K, s->dZ.p, N, 1);
+    gw.C = s->g.p + s->woff[l];
+    SHIP(s, fdt_launch_gemm_bf16(&gw, s->stream));
+    SHIP(s, fdt_launch_mlp_ew(1, s->dZ.p, nullptr, s->g.p + s->boff[l], 0, b, N, s->stream));
+    if (l == 0) break;
+    // dA = dZ . W^T  (b x K), then dZ = dA * gelu'(Z[l-1])
+    GemmArgs ga = gemm(b, K, N, s->dZ.p, N, 1, s->wb.p + s->woff[l], 1, N);
+    ga.C = s->dA.p;
+    SHIP(s, fdt_launch_gemm_bf16(&ga, s->stream));
+    SHIP(s, fdt_launch_mlp_ew(0, s->dA.p, s->Z[l - 1].p, s->dZ.p, (size_t)b * K, 0, 0, s->stream));
+  }
+  if (loss) {
+    std::vector<float> part(nb);
+    SHIP(s, hipMemcpyAsync(part.data(), s->loss_part.p, sizeof(float) * nb, hipMemcpyDeviceToHost, s->stream));
+    SHIP(s, hipStreamSynchronize(s->stream));
+    double t = 0.0;
+    for (float x : part) t += x;
+    *loss = (float)(t / b);
+  }
+  return FDT_OK;
+}
+
+extern "C" {
+
+fdt_surrogate* fdt_surrogate_create(int device, int n_in, uint64_t seed) {
+  if (n_in < 7 || n_in > MAX_N) return nullptr;
+  fdt_surrogate* s = new fdt_surrogate;
+  s->device = device;
+  s->n_in = n_in;
+  const int d[SG_LAYERS + 1] = {n_in, SG_HID0, SG_HID1, SG_HID2, SG_OUT};
+  size_t off = 0;
+  for (int l = 0; l <= SG_LAYERS; ++l) s->dims[l] = d[l];
+  for (int l = 0; l < SG_LAYERS; ++l) {
+    s->woff[l] = off;
+    off += (size_t)d[l] * d[l + 1];
+    s->boff[l] = off;
+    off += d[l + 1];
+  }
+  s->nw = off;
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&s->stream) != hipSuccess || s->w.ensure(off) ||
+      s->m.ensure(off) || s->v.ensure(off) || s->g.ensure(off) || s->wb.ensure(off)) {
+    fdt_surrogate_destroy(s);
+    return nullptr;
+  }
+  // He-normal weights (std sqrt(2/fan_in)) for the GELU layers, sqrt(1/fan_in) for the output, zero biases
+  std::vector<float> h(off, 0.0f);
+  std::mt19937_64 gen(seed);
+  for (int l = 0; l < SG_LAYERS; ++l) {
+    std::normal_distribution<float> nd(0.0f, std::sqrt((l == SG_LAYERS - 1 ? 1.0f : 2.0f) / d[l]));
+    for (size_t i = 0; i < (size_t)d[l] * d[l + 1]; ++i) h[s->woff[l] + i] = nd(gen);
+  }
+  if (fdt_surrogate_load(s, h.data(), off) != FDT_OK) {
+    fdt_surrogate_destroy(s);
+    return nullptr;
+  }
+  return s;
+}
+
+void fdt_surrogate_destroy(fdt_surrogate* s) {
+  if (!s) return;
+  (void)hipSetDevice(s->device);
+  if (s->stream) (void)hipStreamSynchronize(s->stream);
+  DevBuf<float>* fb[] = {&s->w, &s->m, &s->v, &s->g, &s->T, &s->bvals, &s->dA, &s->loss_part, &s->S0};
+  for (auto* b : fb) b->release();
+  s->wb.release(); s->X.release(); s->dZ.release(); s->Xb.release();
+  for (int l = 0; l < SG_LAYERS; ++l) { s->Z[l].release(); s->A[l].release(); }
+  if (s->stream) (void)hipStreamDestroy(s->stream);
+  delete s;
+}
+
+const char* fdt_surrogate_last_error(const fdt_surrogate* s) { return s ? s->err.c_str() : "null surrogate"; }
+
+size_t fdt_surrogate_num_weights(const fdt_surrogate* s) { return s ? s->nw : 0; }
+
+int fdt_surrogate_load(fdt_surrogate* s, const float* w, size_t n) {
+  if (!s || !w) return FDT_EINVAL;
+  if (n != s->nw) return serr(s, FDT_EINVAL, "%zu weights given, the network has %zu", n, s->nw);
+  (void)hipSetDevice(s->device);
+  SHIP(s, hipMemcpyAsync(s->w.p, w, sizeof(float) * n, hipMemcpyHostToDevice, s->stream));
+  SHIP(s, fdt_launch_mlp_ew(2, s->w.p, nullptr, s->wb.p, n, 0, 0, s->stream));
+  SHIP(s, hipMemsetAsync(s->m.p, 0, sizeof(float) * n, s->stream));
+  SHIP(s, hipMemsetAsync(s->v.p, 0, sizeof(float) * n, s->stream));
+  SHIP(s, hipStreamSynchronize(s->stream));
+  s->step = 0;
+  return FDT_OK;
+}
+
+int fdt_surrogate_save(fdt_surrogate* s, float* w, size_t n) {
+  if (!s || !w) return FDT_EINVAL;
+  if (n != s->nw) return serr(s, FDT_EINVAL, "buffer of %zu, the network has %zu weights", n, s->nw);
+  (void)hipSetDevice(s->device);
+  SHIP(s, hipMemcpyAsync(w, s->w.p, sizeof(float) * n, hipMemcpyDeviceToHost, s->stream));
+  SHIP(s, hipStreamSynchronize(s->stream));
+  return FDT_OK;
+}
+
+int fdt_surrogate_set_training_data(fdt_surrogate* s, const float* signals, const float* bvals, const float* targets,
+                                    int n) {
+  if (!s || !signals || !bvals || !targets || n <= 0) return FDT_EINVAL;
+  (void)hipSetDevice(s->device);
+  const int N = s->n_in;
+  DevBuf<float> raw;
+  SHIP(s, raw.ensure((size_t)n * N));
+  SHIP(s, s->bvals.ensure(N));
+  SHIP(s, s->X.ensure((size_t)n * N));
+  SHIP(s, s->T.ensure((size_t)n * SG_OUT));
+  SHIP(s, hipMemcpyAsync(raw.p, signals, sizeof(float) * n * N, hipMemcpyHostToDevice, s->stream));
+  SHIP(s, hipMemcpyAsync(s->bvals.p, bvals, sizeof(float) * N, hipMemcpyHostToDevice, s->stream));
+  SHIP(s, hipMemcpyAsync(s->T.p, targets, sizeof(float) * n * SG_OUT, hipMemcpyHostToDevice, s->stream));
+  SHIP(s, fdt_launch_normalize(raw.p, N, n, s->bvals.p, 50.0f, s->X.p, nullptr, s->stream));
+  SHIP(s, hipStreamSynchronize(s->stream));
+  raw.release();
+  s->ntrain = n;
+  return FDT_OK;
+}
+
+// gradients of the mean loss over training rows [first, first + batch) (no update)
+int fdt_surrogate_gradients(fdt_surrogate* s, int first, int batch, float* grads, float* loss) {
+  if (!s || batch <= 0 || first < 0 || first + batch > s->ntrain) return serr(s, FDT_EINVAL, "bad batch");
+  (void)hipSetDevice(s->device);
+  if (int rc = sg_reserve(s, batch)) return rc;
+  const uint16_t* X = s->X.p + (size_t)first * s->n_in;
+  if (int rc = sg_forward(s, X, batch, true)) return rc;
+  if (int rc = sg_backward(s, X, s->T.p + (size_t)first * SG_OUT, batch, loss)) return rc;
+  if (grads) {
+    SHIP(s, hipMemcpyAsync(grads, s->g.p, sizeof(float) * s->nw, hipMemcpyDeviceToHost, s->stream));
+    SHIP(s, hipStreamSynchronize(s->stream));
+  }
+  return FDT_OK;
+}
+
+// one Adam step on training rows [first, first + batch)
+int fdt_surrogate_train_step(fdt_surrogate* s, int first, int batch, float lr, float* loss) {
+  if (int rc = fdt_surrogate_gradients(s, first, batch, nullptr, loss)) return rc;
+  ++s->step;
+  SHIP(s, fdt_launch_adam(s->w.p, s->m.p, s->v.p, s->g.p, s->wb.p, s->nw, lr, 0.9f, 0.999f, 1e-8f, s->step, s->stream));
+  if (!loss) SHIP(s, hipStreamSynchronize(s->stream));
+  return FDT_OK;
+}
+
+// network outputs for host signals [n][n_in] (tests, evaluation)
+int fdt_surrogate_forward(fdt_surrogate* s, const float* signals, const float* bvals, int n, float* out) {
+  if (!s || !signals || !bvals || !out || n <= 0) return FDT_EINVAL;
+  (void)hipSetDevice(s->device);
+  const int N = s->n_in;
+  DevBuf<float> raw;
+  SHIP(s, raw.ensure((size_t)n * N));
+  SHIP(s, s->bvals.ensure(N));
+  SHIP(s, hipMemcpyAsync(raw.p, signals, sizeof(float) * n * N, hipMemcpyHostToDevice, s->stream));
+  SHIP(s, hipMemcpyAsync(s->bvals.p, bvals, sizeof(float) * N, hipMemcpyHostToDevice, s->stream));
+  const int chunk = 65536;
+  if (int rc = sg_reserve(s, std::min(n, chunk))) return rc;
+  for (int c0 = 0; c0 < n; c0 += chunk) {
+    const int b = std::min(chunk, n - c0);
+    SHIP(s, fdt_launch_normalize(raw.p + (size_t)c0 * N, N, b, s->bvals.p, 50.0f, s->Xb.p, nullptr, s->stream));
+    if (int rc = sg_forward(s, s->Xb.p, b, false)) return rc;
+    SHIP(s, hipMemcpyAsync(out + (size_t)c0 * SG_OUT, s->Z[SG_LAYERS - 1].p, sizeof(float) * b * SG_OUT,
+                           hipMemcpyDeviceToHost, s->stream));
+  }
+  SHIP(s, hipStreamSynchronize(s->stream));
+  raw.release();
+  return FDT_OK;
+}
+
+}  // extern "C"
+
+// Network start values and proposal widths for the context's resident voxels
+// (chunks of 64k); also the predicted direction of fibre 1 per voxel.
+static int sg_infer(fdt_surrogate* s, fdt_ctx* c, DevBuf<float>& dir) {
+  const fdt_opts& o = c->o;
+  const int np = np_of(o), nv = c->nvox;
+  if (s->n_in != c->N) return serr(s, FDT_EINVAL, "network input %d, acquisition %d", s->n_in, c->N);
+  if (o.rician || o.f0) return serr(s, FDT_EUNSUP, "the learned start supports Gaussian noise without --f0");
+  SHIP(s, c->d_params.ensure((size_t)std::max(nv, 1) * np));
+  SHIP(s, c->d_widths_in.ensure((size_t)std::max(nv, 1) * 16));
+  SHIP(s, dir.ensure((size_t)std::max(nv, 1) * 3));
+  SHIP(s, c->d_tau.ensure(std::max(nv, 1)));
+  const int chunk = 65536;
+  if (int rc = sg_reserve(s, std::min(std::max(nv, 1), chunk))) return rc;
+  for (int c0 = 0; c0 < nv; c0 += chunk) {
+    const int b = std::min(chunk, nv - c0);
+    SHIP(s, fdt_launch_normalize(c->d_data.p + (size_t)c0 * c->N, c->N, b, c->d_bvals.p, 50.0f, s->Xb.p, s->S0.p,
+                                 c->stream));
+    // the network runs on the context's stream
+    hipStream_t keep = s->stream;
+    s->stream = c->stream;
+    int rc = sg_forward(s, s->Xb.p, b, false);
+    s->stream = keep;
+    if (rc) return rc;
+    SgDecodeArgs d{};
+    d.O = s->Z[SG_LAYERS - 1].p;
+    d.S0 = s->S0.p;
+    d.nvox = b;
+    d.model = o.model;
+    d.K = o.nfibres;
+    d.f0 = o.f0;
+    d.np = np;
+    d.params = c->d_params.p + (size_t)c0 * np;
+    d.widths = c->d_widths_in.p + (size_t)c0 * 16;
+    d.dir = dir.p + (size_t)c0 * 3;
+    SHIP(s, fdt_launch_sg_decode(&d, c->stream));
+  }
+  return FDT_OK;
+}
+
+extern "C" {
+
+int fdt_surrogate_infer(fdt_surrogate* s, fdt_ctx* c) {
+  if (!s || !c) return FDT_EINVAL;
+  (void)hipSetDevice(c->device);
+  DevBuf<float> dir;
+  if (int rc = sg_infer(s, c, dir)) return rc;
+  SHIP(s, hipStreamSynchronize(c->stream));
+  dir.release();
+  c->have_params = true;
+  c->use_widths_in = true;
+  c->have_samples = c->have_finish = false;
+  return FDT_OK;
+}
+
+// Learned start + short burn-in, with the plain path as fallback: the network
+// gives start values and proposal widths, the chains run nburn_short burn-in
+// sweeps and the recorded ones; voxels whose final sum of squares exceeds
+// chi2_max * N * sigma^2 (sigma^2: variance of the b=0 volumes) or whose
+// posterior mean dyad of fibre 1 lies more than ang_max_deg from the predicted
+// direction are fitted with the plain LM and rerun with the configured
+// burn-in.  *n_fallback receives the number of rerun voxels.
+int fdt_run_surrogate(fdt_ctx* c, fdt_surrogate* s, uint64_t vox_offset, int nburn_short, float chi2_max,
+                      float ang_max_deg, int* n_fallback) {
+  if (!c || !s) return FDT_EINVAL;
+  if (nburn_short < 0) return set_err(c, FDT_EINVAL, "nburn_short=%d", nburn_short);
+  (void)hipSetDevice(c->device);
+  const fdt_opts& o = c->o;
+  const int nv = c->nvox, K = o.nfibres;
+  DevBuf<float> dir;
+  HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
+  if (int rc = sg_infer(s, c, dir)) return set_err(c, rc, "%s", s->err.c_str());
+  HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
+  c->timed_fit = true;
+  c->have_params = true;
+  // short chains from the learned start
+  c->use_widths_in = true;
+  McmcArgs a;
+  mcmc_launch_fn fn = nullptr;
+  if (int rc = mcmc_prepare(c, vox_offset, &a, &fn)) return rc;
+  a.nburn = nburn_short;
+  HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
+  HIPCHK(c, fn(&a, K, 0, c->stream));
+  c->have_samples = true;
+  if (int rc = finish_async(c)) return rc;
+  // fallback list
+  HIPCHK(c, c->d_queue.ensure(NQUEUE));
+  HIPCHK(c, c->d_list.ensure(std::max(nv, 1)));
+  HIPCHK(c, fdt_launch_zero_ints(c->d_queue.p, NQUEUE, c->stream));
+  SgFallbackArgs fa{};
+  fa.data = c->d_data.p;
+  fa.N = c->N;
+  fa.nvox = nv;
+  fa.bvals = c->d_bvals.p;
+  fa.b0_max = 50.0f;
+  fa.energy = c->d_energy.p;
+  fa.dyads = c->m_dyads.p;
+  fa.K = K;
+  HIPCHK(c, c->d_refit.ensure(std::max(nv, 1)));
+  fa.flag = c->d_refit.p;
+  fa.dir = dir.p;
+  fa.chi2_max = chi2_max;
+  fa.cos_min = std::cos(ang_max_deg * 3.14159265358979f / 180.0f);
+  fa.list = c->d_list.p;
+  fa.count = c->d_queue.p + 7;
+  HIPCHK(c, fdt_launch_sg_fallback(&fa, c->stream));
+  int nfb = 0;
+  HIPCHK(c, hipMemcpyAsync(&nfb, c->d_queue.p + 7, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (n_fallback) *n_fallback = nfb;
+  if (nfb > 0) {
+    // plain LM on the listed voxels (stage 1 from the tensor, then PVM_multi for model >= 2)
+    const int np = np_of(o), np1 = 2 + 3 * K;
+    int* q = c->d_queue.p;
+    HIPCHK(c, fdt_launch_zero_ints(q, 4, c->stream));
+    LmArgs l = base_lm_args(c);
+    l.list = c->d_list.p;
+    l.count = c->d_queue.p + 7;
+    l.init = INIT_DTI;
+    l.queue = q;
+    const int k1 = o.model == 1 ? (o.nonlin ? KIND_SINGLE : KIND_SINGLE_C) : KIND_SINGLE;
+    if (o.model == 1) {
+      l.out = c->d_params.p;
+      l.out_np = np;
+      HIPCHK(c, launch_lm(l, K, o.lm_fp64 ? 1 : 0, k1, 0, c->stream));
+    } else {
+      HIPCHK(c, c->d_s1.ensure((size_t)nv * np1));
+      l.out = c->d_s1.p;
+      l.out_np = np1;
+      HIPCHK(c, launch_lm(l, K, o.lm_fp64 ? 1 : 0, k1, 0, c->stream));
+      LmArgs m = l;
+      m.init = INIT_STAGE1;
+      m.in = c->d_s1.p;
+      m.in_np = np1;
+      m.G = o.model == 3 ? 2 : 0;
+      m.queue = q + 1;
+      m.out = c->d_params.p;
+      m.out_np = np;
+      HIPCHK(c, launch_lm(m, K, o.lm_fp64 ? 1 : 0, KIND_MULTI, 0, c->stream));
+    }
+    // their chains with the configured burn-in and default widths
+    McmcArgs b = a;
+    b.nburn = o.nburn;
+    b.widths_in = nullptr;
+    b.list = c->d_list.p;
+    b.nvox = nfb;
+    HIPCHK(c, fn(&b, K, 0, c->stream));
+    if (int rc = finish_async(c)) return rc;
+  }
+  HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
+  c->timed_mcmc = true;
+  c->timed_finish = false;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  dir.release();
+  c->use_widths_in = false;
+  c->have_finish = true;
+  return FDT_OK;
+}
+
+int fdt_get_fallback_flags(fdt_ctx* c, unsigned char* flags) {
+  if (!c || !flags) return FDT_EINVAL;
+  if (!c->d_refit.p || c->d_refit.n < (size_t)c->nvox) return set_err(c, FDT_ESTATE, "no learned-start run");
+  (void)hipSetDevice(c->device);
+  HIPCHK(c, hipMemcpyAsync(flags, c->d_refit.p, c->nvox, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return FDT_OK;
+}
+
+int fdt_get_widths(fdt_ctx* c, float* widths) {
+  if (!c || !widths) return FDT_EINVAL;
+  if (!c->have_samples) return set_err(c, FDT_ESTATE, "no chains have run");
+  (void)hipSetDevice(c->device);
+  HIPCHK(c, hipMemcpyAsync(widths, c->d_widths_out.p, sizeof(float) * c->nvox * 16, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
   return FDT_OK;
 }
 

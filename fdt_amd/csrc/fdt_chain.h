@@ -652,6 +652,10 @@ FDT_DEV void run_chain(const McmcArgs& a, int vox, const ChainLds& L) {
       pr = (val <= 0.0f || val >= 1.0f) ? 0.0f : a.fudge * (lk ? flog(val) : 0.0f);
     }
   }
+  if (a.widths_in) {
+    float w = a.widths_in[(size_t)vox * 16 + rl];
+    if (w > 0.0f) wd = w;
+  }
   c.sval = val;
   c.sprior = pr;
   c.sprop = wd;
@@ -690,7 +694,7 @@ FDT_DEV void run_chain(const McmcArgs& a, int vox, const ChainLds& L) {
 
   // record destination of the lane's slot (:1326-1339)
   float* rec_base = nullptr;
-  size_t rec_stride = (size_t)a.nvox;  // per sample
+  size_t rec_stride = (size_t)a.vstride;  // per sample
   if (rl == SL_F0) rec_base = a.f0s;
   else if (rl == SL_TAU) rec_base = a.taus;
   else if (rl == SL_D) rec_base = a.d;
@@ -700,7 +704,7 @@ FDT_DEV void run_chain(const McmcArgs& a, int vox, const ChainLds& L) {
   else if (rl < SL_FIB + 3 * K) {
     int k = (rl - SL_FIB) / 3, w = (rl - SL_FIB) % 3;
     float* arr = w == 0 ? a.th : (w == 1 ? a.ph : a.f);
-    if (arr) rec_base = arr + (size_t)k * a.nsamples * a.nvox;
+    if (arr) rec_base = arr + (size_t)k * a.nsamples * a.vstride;
   }
   if (rec_base) rec_base += vox;
 
@@ -738,6 +742,7 @@ FDT_DEV void run_chain(const McmcArgs& a, int vox, const ChainLds& L) {
     float tau = c.template v<SL_TAU>();
     if (rl == 0) a.energy[vox] = RIC ? (float)(-(double)a.N * (double)flog(tau) - c.like) : (float)c.like;
   }
+  if (a.widths_out) a.widths_out[(size_t)vox * 16 + rl] = c.sprop;
 }
 
 

@@ -15,8 +15,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VPL <= 18 ?
   const ChainLds L{sacq, sba, ssh, sshb, siso};
   stage_acq<VPL>(a, L);
   __syncthreads();
-  const int vox = blockIdx.x * 16 + (threadIdx.x >> 4);
-  if (vox < a.nvox) run_chain<K, VPL, RIC>(a, vox, L);
+  const int i = blockIdx.x * 16 + (threadIdx.x >> 4);
+  if (i < a.nvox) run_chain<K, VPL, RIC>(a, a.list ? a.list[i] : i, L);
 }
 
 }  // namespace fdt

@@ -87,10 +87,12 @@ CHUNK = 16384
 def voxels(name: str, nvox: int, sub: int = 0, with_truth: bool = True):
     """Returns dict(bvals, bvecs, data [nvox, N] float32 voxel-major, truth...).
 
-    Voxels are drawn in fixed blocks of CHUNK, block c from Philox stream
-    (name, sub * 4096 + c): the result is a pure function of (name, nvox, sub),
-    and the first c*CHUNK voxels do not depend on nvox."""
-    acq_g = rng(name, 1000 + sub)
+    The acquisition depends on `name` only.  Voxels are drawn in fixed blocks
+    of CHUNK, block c from Philox stream (name, 2000 + sub * 4096 + c): the
+    result is a pure function of (name, nvox, sub), the first c*CHUNK voxels
+    do not depend on nvox, and sub > 0 gives further voxels of the same
+    distribution on the same acquisition (training sets)."""
+    acq_g = rng(name, 1000)
     n_b0, shells = ACQ[name]
     bvals, bvecs = acquisition(n_b0, shells, acq_g)
     parts = []

@@ -206,6 +206,48 @@ int fdt_queue_autotune(fdt_queue* q, int jumps, int sample_vox, float* ms_best);
 /* Duration of the last fdt_run_queue (ms) and the tiles in use. */
 int fdt_queue_timing(fdt_queue* q, float* ms_last, int* rows_per_wg, int* vpl_of_job, int njobs);
 
+/* Final proposal width of every voxel and slot after fdt_mcmc / fdt_run:
+ * widths [nvox][16] (slots f0, tau, d, d_std, R, S0, th_k, ph_k, f_k).  No
+ * counterpart in the reference (its widths stay in FibreGPU,
+ * CUDA/fibre_gpu.h:12-83); they label the training set of the learned start. */
+int fdt_get_widths(fdt_ctx* c, float* widths);
+
+/* ------------------------------------------------------ learned start
+ * An MLP N -> 512 -> 512 -> 256 -> 28 (GELU, BF16 weights and activations,
+ * FP32 accumulation) maps a voxel's signal over its mean b=0 signal to start
+ * values (3 f, 3 directions, d, d_std) and 14 log proposal widths.  It is
+ * trained on the GPU on labels from this engine's own LM + MCMC.  No
+ * counterpart in the reference. */
+typedef struct fdt_surrogate fdt_surrogate;
+fdt_surrogate* fdt_surrogate_create(int device, int n_in, uint64_t seed);
+void fdt_surrogate_destroy(fdt_surrogate* s);
+const char* fdt_surrogate_last_error(const fdt_surrogate* s);
+size_t fdt_surrogate_num_weights(const fdt_surrogate* s);
+/* flat FP32 weights: W1 [N][512], b1, W2 [512][512], b2, W3 [512][256], b3, W4 [256][28], b4 */
+int fdt_surrogate_load(fdt_surrogate* s, const float* w, size_t n);
+int fdt_surrogate_save(fdt_surrogate* s, float* w, size_t n);
+/* training set: raw signals [n][N] (voxel-major), bvals [N], targets [n][28]:
+ * f_1..3, dyads x_1..3 (3 each), 1000 d, 1000 d_std, log widths of slots
+ * f0, d, d_std, R, S0, th1, ph1, f1, th2, ph2, f2, th3, ph3, f3 (>= 1e29: unused) */
+int fdt_surrogate_set_training_data(fdt_surrogate* s, const float* signals, const float* bvals, const float* targets,
+                                    int n);
+/* gradients (flat, as the weights) of the mean loss over rows [first, first+batch) */
+int fdt_surrogate_gradients(fdt_surrogate* s, int first, int batch, float* grads, float* loss);
+/* one Adam step (lr; beta 0.9/0.999) on rows [first, first+batch) */
+int fdt_surrogate_train_step(fdt_surrogate* s, int first, int batch, float lr, float* loss);
+/* outputs [n][28] for raw signals [n][N] */
+int fdt_surrogate_forward(fdt_surrogate* s, const float* signals, const float* bvals, int n, float* out);
+/* start values and proposal widths of the context's resident voxels from the
+ * network (replaces fdt_fit) */
+int fdt_surrogate_infer(fdt_surrogate* s, fdt_ctx* c);
+/* learned start + nburn_short burn-in, fallback of suspicious voxels to the
+ * plain LM + configured burn-in, finishing; *n_fallback = rerun voxels */
+int fdt_run_surrogate(fdt_ctx* c, fdt_surrogate* s, uint64_t vox_offset, int nburn_short, float chi2_max,
+                      float ang_max_deg, int* n_fallback);
+/* why each voxel of the last fdt_run_surrogate was rerun: [nvox] bit 1 sum of
+ * squares, bit 2 direction (0: kept) */
+int fdt_get_fallback_flags(fdt_ctx* c, unsigned char* flags);
+
 #ifdef __cplusplus
 }
 #endif
