@@ -1389,7 +1389,7 @@ static int sg_infer(fdt_surrogate* s, fdt_ctx* c, DevBuf<float>& dir) {
   const fdt_opts& o = c->o;
   const int np = np_of(o), nv = c->nvox;
   if (s->n_in != c->N) return serr(s, FDT_EINVAL, "network input %d, acquisition %d", s->n_in, c->N);
-  if (o.rician || o.f0) return serr(s, FDT_EUNSUP, "the learned start supports Gaussian noise without --f0");
+  if (o.f0) return serr(s, FDT_EUNSUP, "the learned start does not model --f0");
   SHIP(s, c->d_params.ensure((size_t)std::max(nv, 1) * np));
   SHIP(s, c->d_widths_in.ensure((size_t)std::max(nv, 1) * 16));
   SHIP(s, dir.ensure((size_t)std::max(nv, 1) * 3));
@@ -1418,6 +1418,21 @@ static int sg_infer(fdt_surrogate* s, fdt_ctx* c, DevBuf<float>& dir) {
     d.widths = c->d_widths_in.p + (size_t)c0 * 16;
     d.dir = dir.p + (size_t)c0 * 3;
     SHIP(s, fdt_launch_sg_decode(&d, c->stream));
+  }
+  if (o.rician) {
+    // Rician precision of the learned start: 1/var of its residuals, as after the LM
+    TauArgs t{};
+    t.data = c->d_data.p;
+    t.N = c->N;
+    t.nvox = nv;
+    t.acq = c->d_acq.p;
+    t.params = c->d_params.p;
+    t.np = np;
+    t.model = o.model;
+    t.f0 = 0;
+    t.R = o.R_prior_mean;
+    t.tau = c->d_tau.p;
+    SHIP(s, launch_tau(t, o.nfibres, 0, c->stream));
   }
   return FDT_OK;
 }
@@ -1464,7 +1479,7 @@ int fdt_run_surrogate(fdt_ctx* c, fdt_surrogate* s, uint64_t vox_offset, int nbu
   if (int rc = mcmc_prepare(c, vox_offset, &a, &fn)) return rc;
   a.nburn = nburn_short;
   HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
-  HIPCHK(c, fn(&a, K, 0, c->stream));
+  HIPCHK(c, fn(&a, K, o.rician, c->stream));
   c->have_samples = true;
   if (int rc = finish_async(c)) return rc;
   // fallback list
@@ -1483,7 +1498,8 @@ int fdt_run_surrogate(fdt_ctx* c, fdt_surrogate* s, uint64_t vox_offset, int nbu
   HIPCHK(c, c->d_refit.ensure(std::max(nv, 1)));
   fa.flag = c->d_refit.p;
   fa.dir = dir.p;
-  fa.chi2_max = chi2_max;
+  // the sum-of-squares test needs Gaussian energies; Rician chains use the direction test only
+  fa.chi2_max = o.rician ? 3.0e38f : chi2_max;
   fa.cos_min = std::cos(ang_max_deg * 3.14159265358979f / 180.0f);
   fa.list = c->d_list.p;
   fa.count = c->d_queue.p + 7;
@@ -1498,6 +1514,7 @@ int fdt_run_surrogate(fdt_ctx* c, fdt_surrogate* s, uint64_t vox_offset, int nbu
     int* q = c->d_queue.p;
     HIPCHK(c, fdt_launch_zero_ints(q, 4, c->stream));
     LmArgs l = base_lm_args(c);
+    l.rician = o.rician;
     l.list = c->d_list.p;
     l.count = c->d_queue.p + 7;
     l.init = INIT_DTI;
@@ -1522,13 +1539,27 @@ int fdt_run_surrogate(fdt_ctx* c, fdt_surrogate* s, uint64_t vox_offset, int nbu
       m.out_np = np;
       HIPCHK(c, launch_lm(m, K, o.lm_fp64 ? 1 : 0, KIND_MULTI, 0, c->stream));
     }
+    if (o.rician) {
+      // tau from the LM fits (recomputed for every voxel; only the rerun ones read it again)
+      TauArgs t{};
+      t.data = c->d_data.p;
+      t.N = c->N;
+      t.nvox = nv;
+      t.acq = c->d_acq.p;
+      t.params = c->d_params.p;
+      t.np = np;
+      t.model = o.model;
+      t.R = o.R_prior_mean;
+      t.tau = c->d_tau.p;
+      HIPCHK(c, launch_tau(t, K, o.lm_fp64 ? 1 : 0, c->stream));
+    }
     // their chains with the configured burn-in and default widths
     McmcArgs b = a;
     b.nburn = o.nburn;
     b.widths_in = nullptr;
     b.list = c->d_list.p;
     b.nvox = nfb;
-    HIPCHK(c, fn(&b, K, 0, c->stream));
+    HIPCHK(c, fn(&b, K, o.rician, c->stream));
     if (int rc = finish_async(c)) return rc;
   }
   HIPCHK(c, hipEventRecord(c->ev[3], c->stream));

@@ -52,6 +52,7 @@ ACQ = {
     "c2_69": (5, [(1000.0, 64)]),
     "c2_145": (10, [(1000.0, 45), (2000.0, 45), (3000.0, 45)]),
     "c2_288": (18, [(1000.0, 90), (2000.0, 90), (3000.0, 90)]),
+    "c4": (18, [(1000.0, 90), (2000.0, 90), (3000.0, 90)]),
 }
 
 
@@ -65,6 +66,34 @@ def ellipsoid_count(shape, axes) -> int:
 
 
 C1_NVOX = ellipsoid_count((110, 110, 72), (48, 48, 26))
+C4_GRID, C4_AXES = (145, 174, 145), (60, 75, 48)
+C4_NVOX = ellipsoid_count(C4_GRID, C4_AXES)
+C4_R = 0.13  # zeppelin ratio of the model-3 truth (the R prior mean)
+_C4_POS = None
+
+
+def c4_positions() -> np.ndarray:
+    """grid coordinates [C4_NVOX][3] of the high-resolution mask, in mask order"""
+    global _C4_POS
+    if _C4_POS is None:
+        c = [(n - 1) / 2.0 for n in C4_GRID]
+        x, y, z = np.meshgrid(*[np.arange(n) for n in C4_GRID], indexing="ij")
+        r = ((x - c[0]) / C4_AXES[0]) ** 2 + ((y - c[1]) / C4_AXES[1]) ** 2 + ((z - c[2]) / C4_AXES[2]) ** 2
+        _C4_POS = np.stack(np.nonzero(r <= 1.0), 1).astype(np.float64)
+    return _C4_POS
+
+
+def c4_snr(pos: np.ndarray) -> np.ndarray:
+    """SNR field in [10, 30]: trilinear interpolation of seeded values on the 8 corners of the grid"""
+    corner = rng("c4", 999).uniform(10.0, 30.0, (2, 2, 2))
+    u = pos / (np.array(C4_GRID) - 1.0)
+    out = np.zeros(len(pos))
+    for i in (0, 1):
+        for j in (0, 1):
+            for k in (0, 1):
+                w = (u[:, 0] if i else 1 - u[:, 0]) * (u[:, 1] if j else 1 - u[:, 1]) * (u[:, 2] if k else 1 - u[:, 2])
+                out += w * corner[i, j, k]
+    return out
 
 
 def _perp_at_angle(u: np.ndarray, ang: np.ndarray, g: np.random.Generator) -> np.ndarray:
@@ -97,15 +126,20 @@ def voxels(name: str, nvox: int, sub: int = 0, with_truth: bool = True):
     bvals, bvecs = acquisition(n_b0, shells, acq_g)
     parts = []
     for c0 in range(0, nvox, CHUNK):
+        extra = {}
+        if name == "c4":  # SNR from the voxel's place in the high-resolution mask (wrapping for training sets)
+            pos = c4_positions()
+            idx = (np.arange(c0, min(nvox, c0 + CHUNK)) + sub * 104729) % len(pos)
+            extra["snr"] = c4_snr(pos[idx])
         parts.append(_voxel_block(name, min(CHUNK, nvox - c0), rng(name, 2000 + sub * 4096 + c0 // CHUNK), bvals, bvecs,
-                                  with_truth))
+                                  with_truth, **extra))
     out = {k: (np.concatenate([p[k] for p in parts]) if isinstance(parts[0][k], np.ndarray) else parts[0][k])
            for k in parts[0]}
     out.update(bvals=bvals, bvecs=bvecs)
     return out
 
 
-def _voxel_block(name, nvox, g, bvals, bvecs, with_truth):
+def _voxel_block(name, nvox, g, bvals, bvecs, with_truth, snr=None):
     N = bvals.size
     K = 3
     S0 = g.uniform(800.0, 1200.0, nvox)
@@ -133,8 +167,12 @@ def _voxel_block(name, nvox, g, bvals, bvecs, with_truth):
         tot = g.uniform(0.3, 0.8, nvox)
         f = w * tot[:, None]
         dirs[:, 0] = unit_vectors(g, nvox)
+        narrow = g.uniform(size=nvox) < 0.15 if name == "c4" else np.zeros(nvox, bool)
         for k in (1, 2):
-            ang = np.deg2rad(g.uniform(30.0, 90.0, nvox))
+            if name == "c4":  # 15% of voxels cross below 45 degrees
+                ang = np.deg2rad(np.where(narrow, g.uniform(20.0, 45.0, nvox), g.uniform(45.0, 90.0, nvox)))
+            else:
+                ang = np.deg2rad(g.uniform(30.0, 90.0, nvox))
             base = dirs[:, k - 1] if k == 1 else dirs[:, 0]
             v = _perp_at_angle(base, ang, g)
             if k == 2:  # keep the third stick >= 30 deg from the second as well
@@ -143,13 +181,17 @@ def _voxel_block(name, nvox, g, bvals, bvecs, with_truth):
                 v[bad] = np.cross(dirs[bad, 0], dirs[bad, 1])
                 v[bad] /= np.linalg.norm(v[bad], axis=1, keepdims=True)
             dirs[:, k] = v
-        model, snr = 2, 25.0
+        if name == "c4":
+            dstd = np.minimum(dstd, 0.004)
+            model = 3
+        else:
+            model, snr = 2, 25.0
     sig = clean_signal(model, bvals, bvecs, S0, d, dstd, f, dirs)
     sigma = S0 / snr
     n1 = g.standard_normal((nvox, N)) * sigma[:, None]
     n2 = g.standard_normal((nvox, N)) * sigma[:, None]
     data = np.sqrt((sig + n1) ** 2 + n2 ** 2).astype(np.float32)
-    out = dict(data=data, model=model)
+    out = dict(data=data, model=model, snr=np.broadcast_to(np.asarray(snr, dtype=np.float64), (nvox,)).copy())
     if with_truth:
         th, ph = _sph(dirs.reshape(-1, 3))
         out.update(S0=S0, d=d, dstd=dstd, f=f, dirs=dirs, th=th.reshape(nvox, K), ph=ph.reshape(nvox, K),
@@ -165,6 +207,14 @@ def clean_signal(model, bvals, bvecs, S0, d, dstd, f, dirs):
     if model == 1:
         iso = np.exp(-d[:, None] * b)
         an = np.exp(-d[:, None, None] * b[:, None, :] * a)
+    elif model == 3:
+        # gamma-distributed isotropic diffusivity, zeppelin sticks with ratio C4_R
+        # (the MCMC forms of CUDA/runmcmc_kernels.cu:67-95 with R fixed)
+        s2 = dstd ** 2
+        alpha = (d * d / s2)[:, None]
+        iso = np.exp(np.log(d[:, None] / (d[:, None] + b * s2[:, None])) * alpha)
+        R = C4_R
+        an = np.exp(-3.0 * d[:, None, None] / (2 * R + 1) * b[:, None, :] * ((1 - R) * a + R))
     else:
         s2 = dstd ** 2
         alpha = (d * d / s2)[:, None]
