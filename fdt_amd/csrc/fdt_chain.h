@@ -11,9 +11,9 @@
 //   the residual r = y - S0*M (Gaussian) or the data y (Rician),
 //   the mixture M = sum_k f_k s_k + (1 - f0 - sum_k f_k) iso + f0,
 //   the isotropic signal iso and the stick signals s_k;
-// b*(g.x_k)^2, read only by the diffusivity proposals, sits in LDS (one block
-// per wave, lane-interleaved), so that K=3 at 18 volumes per lane fits two
-// waves per SIMD.
+// The candidate stick signals of the diffusivity proposals are parked in LDS
+// (one block per wave, lane-interleaved) so that an accepted move copies them;
+// K=3 at 18 volumes per lane fits two waves per SIMD.
 // A proposal evaluates only what it changes; its likelihood change is
 // summed over the row with DPP adds.  For Gaussian noise a move of every
 // residual by -c*q gives the change of the sum of squares c*(c*sum q^2 -
@@ -133,7 +133,7 @@ struct Chain {
   float M[VPL];
   float iso[VPL];
   float s[K][VPL];
-  float* bal;  // b*(g.x_k)^2 of the lane's volumes, in the wave's LDS block: bal[(k*VPL+t)*64]
+  float* cand;  // candidate stick signals of the diffusivity proposals, wave's LDS block: cand[(k*VPL+t)*64]
   float ell[RIC ? VPL : 1];
   // isotropic signal per b-shell (acquisitions with <= 16 distinct b values):
   // sh[i] is the shell of volume i, shb[l] its b value, isot the row's table
@@ -164,7 +164,22 @@ struct Chain {
     if (rl == SL) { sval = val; scnt += 1; sprior = prn; }
   }
 
-  FDT_DEV float ba(int k, int t) const { return bal[(k * VPL + t) * 64]; }
+  FDT_DEV float& cnd(int k, int t) const { return cand[(k * VPL + t) * 64]; }
+  // unit directions of the row's fibres (from the theta / phi slots)
+  struct Dirs {
+    float x[K][3];
+  };
+  FDT_DEV Dirs dirs() const {
+    Dirs D;
+    sfor<K>([&](auto kc) {
+      constexpr int k = kc;
+      float st, ct, sp, cp;
+      __sincosf(v<TH(k)>(), &st, &ct);
+      __sincosf(v<PH(k)>(), &sp, &cp);
+      D.x[k][0] = st * cp; D.x[k][1] = st * sp; D.x[k][2] = ct;
+    });
+    return D;
+  }
   FDT_DEV float mask_last(int t, float x) const { return (t == VPL - 1 && !last_ok) ? 0.0f : x; }
   // the fractions of the row: fk[k] and f0, their sum
   struct Fr {
@@ -323,13 +338,23 @@ struct Chain {
   }
   // candidate mixture of slot t when the stick signals (NA) and/or the
   // isotropic signal (NI) are re-evaluated with coefficients ca / ci
+  // The candidate stick signals are evaluated from b*(g.x_k)^2 and parked in
+  // LDS, so that an accepted move copies them instead of re-evaluating.
   template <int SA, int SI, bool NA, bool NI, bool SH>
-  FDT_DEV float mix_new(int t, const Fr& q, const SigC& ca, const SigC& ci) const {
-    float b = acq[rl + 16 * t].w;
-    float isn = NI ? iso_new<SI, SH>(t, ci, b) : iso[t];
+  FDT_DEV float mix_new(int t, const Fr& q, const SigC& ca, const SigC& ci, const Dirs& D) const {
+    const float4 g = acq[rl + 16 * t];
+    float isn = NI ? iso_new<SI, SH>(t, ci, g.w) : iso[t];
     float m = fmaf(1.0f - q.sum, isn, q.f0);
 #pragma unroll
-    for (int k = 0; k < K; ++k) m = fmaf(q.fk[k], NA ? sig<SA>(ca, ba(k, t), b) : s[k][t], m);
+    for (int k = 0; k < K; ++k) {
+      float sk = s[k][t];
+      if constexpr (NA) {
+        float dp = fmaf(g.x, D.x[k][0], fmaf(g.y, D.x[k][1], g.z * D.x[k][2]));
+        sk = sig<SA>(ca, g.w * dp * dp, g.w);
+        cnd(k, t) = sk;
+      }
+      m = fmaf(q.fk[k], sk, m);
+    }
     return m;
   }
   template <int SA, int SI, bool NA, bool NI, bool SH>
@@ -338,7 +363,7 @@ struct Chain {
     if (NI) iso[t] = iso_new<SI, SH>(t, ci, b);
     if (NA) {
 #pragma unroll
-      for (int k = 0; k < K; ++k) s[k][t] = sig<SA>(ca, ba(k, t), b);
+      for (int k = 0; k < K; ++k) s[k][t] = cnd(k, t);
     }
   }
   // dispatch over the coefficient forms that occur: model 1 (0,0); model 2
@@ -399,6 +424,7 @@ struct Chain {
     }
     const SigC ca = aniso_coef(model, dn, dsn, Rn), ci = iso_coef(model, dn, dsn);
     const Fr q = fractions();
+    const Dirs D = new_aniso ? dirs() : Dirs{};
     const float S0 = v<SL_S0>(), tau = v<SL_TAU>();
     reload_acq();
     float a = 0.0f;
@@ -409,7 +435,7 @@ struct Chain {
 #pragma unroll
         for (int t = 0; t < VPL; ++t) {
           float mn = mix_new<decltype(SA)::value, decltype(SI)::value, decltype(NA)::value, decltype(NI)::value,
-                             decltype(SH)::value>(t, q, ca, ci);
+                             decltype(SH)::value>(t, q, ca, ci, D);
           if constexpr (RIC) a += mask_last(t, ell_of(r[t], S0 * mn, tau) - ell[t]);
           else qd.add(r[t], mask_last(t, mn - M[t]));
         }
@@ -477,11 +503,6 @@ struct Chain {
         M[t] += dM;
         if constexpr (RIC) ell[t] = en[t]; else r[t] -= S0 * dM;
         s[k][t] = sn[t];
-        // b*(g.x)^2 of the accepted direction, recomputed rather than kept
-        // live through the row sum (the same expression as above)
-        float4 g = acq[rl + 16 * t];
-        float dp = fmaf(g.x, x0, fmaf(g.y, x1, g.z * x2));
-        bal[(k * VPL + t) * 64] = g.w * dp * dp;
       }
       like += (double)a;
       if constexpr (IS_TH) accept<SL>(xn, prn);
@@ -531,12 +552,12 @@ struct Chain {
   }
 };
 
-// LDS of a chain workgroup: the acquisition table, b*(g.x_k)^2 per wave, the
+// LDS of a chain workgroup: the acquisition table, candidate stick signals per wave, the
 // b-shell index of each volume and the shells' b values, the rows' isotropic
 // shell tables
 struct ChainLds {
   float4* acq;
-  float* ba;
+  float* cand;
   int* sh;
   float* shb;
   float* iso;
@@ -566,7 +587,7 @@ FDT_DEV void run_chain(const McmcArgs& a, int vox, const ChainLds& L) {
   C c;
   c.rl = rl;
   c.acq = sacq;
-  c.bal = L.ba + (threadIdx.x >> 6) * (K * VPL * 64) + lane;
+  c.cand = L.cand + (threadIdx.x >> 6) * (K * VPL * 64) + lane;
   c.sh = L.sh;
   c.shb = L.shb;
   c.isot = L.iso + (threadIdx.x >> 4) * 16;
@@ -662,24 +683,17 @@ FDT_DEV void run_chain(const McmcArgs& a, int vox, const ChainLds& L) {
   c.scnt = 0;
   {
     SigC ca = aniso_coef(a.model, d, dstd, Rr), ci = iso_coef(a.model, d, dstd);
-    sfor<K>([&](auto kc) {
-      constexpr int k = kc;
-      float st, ct, sp, cp;
-      __sincosf(c.template v<C::TH(k)>(), &st, &ct);
-      __sincosf(c.template v<C::PH(k)>(), &sp, &cp);
-      float x0 = st * cp, x1 = st * sp, x2 = ct;
+    const typename C::Dirs D = c.dirs();
+    by_kind(ca.kind, [&](auto SK) {
 #pragma unroll
       for (int t = 0; t < VPL; ++t) {
         float4 g = sacq[rl + 16 * t];
-        float dp = fmaf(g.x, x0, fmaf(g.y, x1, g.z * x2));
-        c.bal[(k * VPL + t) * 64] = g.w * dp * dp;
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+          float dp = fmaf(g.x, D.x[k][0], fmaf(g.y, D.x[k][1], g.z * D.x[k][2]));
+          c.s[k][t] = sig<decltype(SK)::value>(ca, g.w * dp * dp, g.w);
+        }
       }
-    });
-    by_kind(ca.kind, [&](auto SK) {
-#pragma unroll
-      for (int t = 0; t < VPL; ++t)
-#pragma unroll
-        for (int k = 0; k < K; ++k) c.s[k][t] = sig<decltype(SK)::value>(ca, c.ba(k, t), sacq[rl + 16 * t].w);
     });
     by_kind(ci.kind, [&](auto SK) {
 #pragma unroll
