@@ -200,9 +200,14 @@ def _voxel_block(name, nvox, g, bvals, bvecs, with_truth, snr=None):
 
 
 def clean_signal(model, bvals, bvecs, S0, d, dstd, f, dirs):
-    """Noise-free ball-and-sticks signal [nvox, N] (float64)."""
-    b = bvals[None, :]
-    dp = np.einsum("vkc,cn->vkn", dirs, bvecs)
+    """Noise-free ball-and-sticks signal [nvox, N] (float64).  bvals [N] and
+    bvecs [3, N] are shared, or [nvox, N] and [nvox, 3, N] per voxel."""
+    if np.ndim(bvecs) == 3:
+        b = np.asarray(bvals)
+        dp = np.einsum("vkc,vcn->vkn", dirs, bvecs)
+    else:
+        b = bvals[None, :]
+        dp = np.einsum("vkc,cn->vkn", dirs, bvecs)
     a = dp * dp
     if model == 1:
         iso = np.exp(-d[:, None] * b)
@@ -222,3 +227,43 @@ def clean_signal(model, bvals, bvecs, S0, d, dstd, f, dirs):
         an = np.exp(np.log(d[:, None, None] / (d[:, None, None] + b[:, None, :] * a * s2[:, None, None])) * alpha[:, :, None])
     fs = f.sum(1)
     return S0[:, None] * ((1.0 - fs)[:, None] * iso + (f[:, :, None] * an).sum(1))
+
+
+def grad_dev(g: np.random.Generator, nvox: int, sd: float = 0.03) -> np.ndarray:
+    """per-voxel gradient-deviation values [nvox][9] (the grad_dev volume of a
+    scanner with gradient nonlinearity): L(i, j) = grad[3j + i], entries ~ N(0, sd)"""
+    return (g.standard_normal((nvox, 9)) * sd).astype(np.float32)
+
+
+def corrected_acquisitions(bvals, bvecs, grad):
+    """per-voxel (bvals [nvox][N], bvecs [nvox][3][N]) seen through the
+    gradient deviations: v -> (I+L) v / |(I+L) v|, b -> b |(I+L) v|^2 for b > 0"""
+    grad = np.asarray(grad, dtype=np.float64)
+    L = grad.reshape(-1, 3, 3).transpose(0, 2, 1)           # L[v, i, j] = grad[v, 3j + i]
+    M = np.eye(3)[None] + L
+    w = np.einsum("vij,jn->vin", M, bvecs)
+    mag = np.linalg.norm(w, axis=1)
+    dw = bvals[None, :] > 0
+    bc = np.where(dw[:, None, :], w / np.where(mag > 0, mag, 1.0)[:, None, :], bvecs[None])
+    bv = np.where(dw, bvals[None, :] * mag ** 2, bvals[None, :])
+    return bv, bc
+
+
+def gradnonlin_voxels(name: str, nvox: int, sd: float = 0.03, clean_only: bool = False):
+    """voxels of workload `name` whose signal was acquired through per-voxel
+    gradient deviations (returned as "grad", [nvox][9] float32): same truth
+    and noise draws as voxels(name, nvox), signal from the corrected
+    acquisitions."""
+    v = voxels(name, nvox)
+    grad = grad_dev(rng(name, 3000), nvox, sd)
+    bv, bc = corrected_acquisitions(v["bvals"], v["bvecs"], grad)
+    model = v["model"]
+    sig = clean_signal(model, bv, bc, v["S0"], v["d"], v["dstd"], v["f"], v["dirs"])
+    g = rng(name, 3001)
+    sigma = v["S0"] / v["snr"]
+    n1 = g.standard_normal(sig.shape) * sigma[:, None]
+    n2 = g.standard_normal(sig.shape) * sigma[:, None]
+    v["clean"] = sig.astype(np.float32)
+    v["data"] = v["clean"] if clean_only else np.sqrt((sig + n1) ** 2 + n2 ** 2).astype(np.float32)
+    v["grad"] = grad
+    return v

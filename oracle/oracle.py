@@ -60,11 +60,12 @@ def lib():
             getattr(L, p + "_model_eval").argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, dp,
                                                      dp, dp, dp, dp, dp, dp, dp]
             getattr(L, p + "_lu_solve").argtypes = [C.c_int, dp, dp, dp]
-            getattr(L, p + "_start").argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, dp, dp, C.c_int, dp, dp]
-            getattr(L, p + "_fit").argtypes = [C.POINTER(OrcOpts), C.c_int, dp, dp, C.c_int, dp, dp, dp, ip]
+            getattr(L, p + "_start").argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, dp, dp, C.c_int, dp, dp, dp]
+            getattr(L, p + "_fit").argtypes = [C.POINTER(OrcOpts), C.c_int, dp, dp, C.c_int, dp, dp, dp, ip, dp]
             getattr(L, p + "_mcmc").argtypes = [C.POINTER(OrcOpts), C.c_int, dp, dp, C.c_int, dp, dp, dp,
-                                               C.c_uint64, C.POINTER(OrcSamples), dp]
+                                               C.c_uint64, C.POINTER(OrcSamples), dp, dp]
         L.orc_dti.argtypes = [C.c_int, dp, dp, C.c_int, dp, dp]
+        L.orc_correct_acq.argtypes = [C.c_int, dp, dp, dp, dp, dp]
         L.orc_pinv.argtypes = [C.c_int, C.c_int, dp, dp]
         L.orc_eig.argtypes = [C.c_int, dp, dp, dp]
         u32p = C.POINTER(C.c_uint32)
@@ -123,15 +124,34 @@ def lm(kind, K, f0, G, Rv, bvals, bvecs, data, start, prec=64):
     return out, iters, cost
 
 
-def start(kind, K, f0, bvals, bvecs, data, prec=64):
+def _grad(grad, nvox):
+    """per-voxel gradient-deviation values [nvox][9] (--gradnonlin) or a null pointer"""
+    if grad is None:
+        return None, C.POINTER(C.c_double)()
+    g = np.ascontiguousarray(grad, dtype=np.float64).reshape(nvox, 9)
+    return g, g.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def correct_acq(bvals, bvecs, g):
+    """corrected (bvals [N], bvecs [3][N]) of one voxel with gradient deviation g [9]"""
+    N = len(bvals)
+    bv_c = np.zeros(N)
+    bc_c = np.zeros((3, N))
+    lib().orc_correct_acq(N, _d(bvals)[1], _d(bvecs)[1], _d(g)[1], bv_c.ctypes.data_as(C.POINTER(C.c_double)),
+                          bc_c.ctypes.data_as(C.POINTER(C.c_double)))
+    return bv_c, bc_c
+
+
+def start(kind, K, f0, bvals, bvecs, data, prec=64, grad=None):
     L = lib()
     data = np.ascontiguousarray(data, dtype=np.float64)
     nvox, N = data.shape
     np_ = n_params(kind, K, f0)
     out = np.zeros((nvox, np_))
     bv, bc = _d(bvals), _d(bvecs)
+    g = _grad(grad, nvox)
     getattr(L, f"orc{prec}_start")(kind, K, int(f0), N, bv[1], bc[1], nvox, _d(data)[1],
-                                   out.ctypes.data_as(C.POINTER(C.c_double)))
+                                   out.ctypes.data_as(C.POINTER(C.c_double)), g[1])
     return out
 
 
@@ -191,7 +211,7 @@ def logI0(x):
     return lib().orc_logI0(float(x))
 
 
-def fit(o: OrcOpts, bvals, bvecs, data, prec=64):
+def fit(o: OrcOpts, bvals, bvecs, data, prec=64, grad=None):
     data = np.ascontiguousarray(data, dtype=np.float64)
     nvox, N = data.shape
     npo = 2 + 3 * o.nfibres + (1 if o.model >= 2 else 0) + (1 if o.f0 else 0)
@@ -202,7 +222,7 @@ def fit(o: OrcOpts, bvals, bvecs, data, prec=64):
     getattr(lib(), f"orc{prec}_fit")(C.byref(o), N, bv[1], bc[1], nvox, data.ctypes.data_as(C.POINTER(C.c_double)),
                                      params.ctypes.data_as(C.POINTER(C.c_double)),
                                      tau.ctypes.data_as(C.POINTER(C.c_double)),
-                                     iters.ctypes.data_as(C.POINTER(C.c_int)))
+                                     iters.ctypes.data_as(C.POINTER(C.c_int)), _grad(grad, nvox)[1])
     return params, tau, iters
 
 
@@ -221,7 +241,7 @@ def alloc_samples(o: OrcOpts, nvox: int):
     return arrs, s
 
 
-def mcmc(o: OrcOpts, bvals, bvecs, data, params, tau=None, vox_offset=0, prec=64):
+def mcmc(o: OrcOpts, bvals, bvecs, data, params, tau=None, vox_offset=0, prec=64, grad=None):
     data = np.ascontiguousarray(data, dtype=np.float64)
     nvox, N = data.shape
     params = np.ascontiguousarray(params, dtype=np.float64)
@@ -231,7 +251,7 @@ def mcmc(o: OrcOpts, bvals, bvecs, data, params, tau=None, vox_offset=0, prec=64
     bv, bc = _d(bvals), _d(bvecs)
     getattr(lib(), f"orc{prec}_mcmc")(C.byref(o), N, bv[1], bc[1], nvox, data.ctypes.data_as(C.POINTER(C.c_double)),
                                       params.ctypes.data_as(C.POINTER(C.c_double)), tp, vox_offset, C.byref(s),
-                                      energy.ctypes.data_as(C.POINTER(C.c_double)))
+                                      energy.ctypes.data_as(C.POINTER(C.c_double)), _grad(grad, nvox)[1])
     arrs["energy"] = energy
     return arrs
 

@@ -27,6 +27,7 @@
 #include <vector>
 #include <algorithm>
 #include <limits>
+#include <memory>
 #ifdef _OPENMP
 #include <omp.h>
 #endif
@@ -260,6 +261,37 @@ struct Acq {
   }
 };
 
+// correct_bvals_bvecs (CUDA/xfibres_gpu.cu:151-170; xfibres.cc:797-817): the
+// voxel's gradient-deviation tensor L (9 values g, L(i,j) = g[3j+i]) maps each
+// diffusion-weighted direction v to (I+L) v; the direction is renormalised and
+// b scaled by |(I+L) v|^2; b <= 0 volumes are left alone.  The products are in
+// double, |.| is rounded to float (the reference's `float mag`), and the
+// results are rounded to float as the reference hands them to its kernels in
+// float host vectors (CUDA/xfibres_gpu.cu:75, 423-427).
+static void correct_acq(int N, const double* bvals, const double* bvecs, const double* g,
+                        double* bvals_c, double* bvecs_c) {
+  double M[3][3];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) M[i][j] = (i == j ? 1.0 : 0.0) + g[3 * j + i];
+  for (int l = 0; l < N; ++l) {
+    double v[3] = {bvecs[l], bvecs[N + l], bvecs[2 * N + l]};
+    double b = bvals[l];
+    if (b > 0) {
+      // left-to-right sums, no fused multiply-add (the build has no -mfma)
+      double w[3];
+      for (int i = 0; i < 3; ++i) w[i] = M[i][0] * v[0] + M[i][1] * v[1] + M[i][2] * v[2];
+      float mag = (float)std::sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+      if (mag != 0.0f)
+        for (int i = 0; i < 3; ++i) w[i] = w[i] / (double)mag;
+      float m2 = mag * mag;
+      b = (double)m2 * b;
+      for (int i = 0; i < 3; ++i) v[i] = w[i];
+    }
+    bvals_c[l] = (double)(float)b;
+    for (int i = 0; i < 3; ++i) bvecs_c[i * N + l] = (double)(float)v[i];
+  }
+}
+
 // MISCMATHS::cart2sph (un-vendored): polar angle from +z, azimuth atan2(y,x);
 // pinned by the forward map (sin th cos ph, sin th sin ph, cos th) round trip.
 static inline void cart2sph(const double v[3], double& th, double& ph) {
@@ -285,6 +317,32 @@ static void dti_amat_pinv(int N, const double* bvals, const double* bvecs, std::
   }
   pinv(N, 7, A.data(), iA.data());
 }
+
+// The acquisition one voxel is fitted with: the shared one, or under
+// --gradnonlin its own corrected copy (per-voxel bvecs/bvals and tensor
+// design, CUDA/xfibres_gpu.cu:403-436, CUDA/diffmodels.cu:57-63;
+// xfibres.cc:909-919).
+template <class R>
+struct VoxAcq {
+  const Acq<R>* acq;
+  const std::vector<double>* iA;
+  std::vector<double> bv, bc, A, iAv;
+  std::unique_ptr<Acq<R>> own;
+  VoxAcq(int N, const double* bvals, const double* bvecs, const double* g, const Acq<R>& acq0,
+         const std::vector<double>& iA0, bool need_pinv = true)
+      : acq(&acq0), iA(&iA0) {
+    if (!g) return;
+    bv.resize(N);
+    bc.resize(3 * (size_t)N);
+    correct_acq(N, bvals, bvecs, g, bv.data(), bc.data());
+    own.reset(new Acq<R>(N, bv.data(), bc.data()));
+    acq = own.get();
+    if (need_pinv) {
+      dti_amat_pinv(N, bv.data(), bc.data(), A, iAv);
+      iA = &iAv;
+    }
+  }
+};
 
 // eigenvector sign convention shared with the GPU product: the component of
 // largest magnitude is made positive (the reference's newmat EigenValues sign
@@ -1313,35 +1371,37 @@ typedef struct orc_samples {
     return 0;                                                                                                 \
   }                                                                                                           \
   int PFX##_start(int kind, int K, int f0, int N, const double* bvals, const double* bvecs, int nvox,         \
-                  const double* data, double* start_out) {                                                    \
-    orc::Acq<RT> acq(N, bvals, bvecs);                                                                         \
-    std::vector<double> A, iA;                                                                                \
-    orc::dti_amat_pinv(N, bvals, bvecs, A, iA);                                                               \
+                  const double* data, double* start_out, const double* grad) {                                \
+    orc::Acq<RT> acq0(N, bvals, bvecs);                                                                        \
+    std::vector<double> A0, iA0;                                                                              \
+    orc::dti_amat_pinv(N, bvals, bvecs, A0, iA0);                                                             \
     int np = 2 + 3 * K + (f0 ? 1 : 0);                                                                        \
     _Pragma("omp parallel for schedule(static)") for (int v = 0; v < nvox; ++v) {                             \
+      orc::VoxAcq<RT> va(N, bvals, bvecs, grad ? grad + (size_t)v * 9 : nullptr, acq0, iA0);                   \
       orc::DtiOut dti;                                                                                        \
-      orc::dti_linfit(N, iA.data(), data + (size_t)v * N, dti);                                               \
+      orc::dti_linfit(N, va.iA->data(), data + (size_t)v * N, dti);                                           \
       RT p[orc::NPMAX];                                                                                        \
       std::vector<RT> y(N);                                                                                    \
       for (int i = 0; i < N; ++i) y[i] = (RT)data[(size_t)v * N + i];                                          \
       if (kind == orc::K_SINGLE) orc::start_single<RT>(dti, K, f0 != 0, p);                                    \
-      else orc::start_single_c<RT>(dti, acq, y.data(), K, f0 != 0, p);                                         \
+      else orc::start_single_c<RT>(dti, *va.acq, y.data(), K, f0 != 0, p);                                     \
       for (int i = 0; i < np; ++i) start_out[(size_t)v * np + i] = (double)p[i];                              \
     }                                                                                                         \
     return np;                                                                                                \
   }                                                                                                           \
   int PFX##_fit(const orc_opts* o, int N, const double* bvals, const double* bvecs, int nvox,                 \
-                const double* data, double* params, double* tau, int* iters) {                                \
-    orc::Acq<RT> acq(N, bvals, bvecs);                                                                         \
-    std::vector<double> A, iA;                                                                                \
-    orc::dti_amat_pinv(N, bvals, bvecs, A, iA);                                                               \
+                const double* data, double* params, double* tau, int* iters, const double* grad) {            \
+    orc::Acq<RT> acq0(N, bvals, bvecs);                                                                        \
+    std::vector<double> A0, iA0;                                                                              \
+    orc::dti_amat_pinv(N, bvals, bvecs, A0, iA0);                                                             \
     int npo = 2 + 3 * o->nfibres + (o->model >= 2 ? 1 : 0) + (o->f0 ? 1 : 0);                                 \
     _Pragma("omp parallel for schedule(dynamic, 4)") for (int v = 0; v < nvox; ++v) {                          \
+      orc::VoxAcq<RT> va(N, bvals, bvecs, grad ? grad + (size_t)v * 9 : nullptr, acq0, iA0);                   \
       std::vector<RT> y(N);                                                                                    \
       for (int i = 0; i < N; ++i) y[i] = (RT)data[(size_t)v * N + i];                                          \
       if (o->rician) orc::remove_nonpositive<RT>(y);                                                           \
       orc::FitOut<RT> fo;                                                                                      \
-      orc::fit_one<RT>(*o, acq, iA, y, fo);                                                                    \
+      orc::fit_one<RT>(*o, *va.acq, *va.iA, y, fo);                                                            \
       for (int i = 0; i < npo; ++i) params[(size_t)v * npo + i] = (double)fo.p[i];                            \
       if (tau) tau[v] = (double)fo.tau;                                                                       \
       if (iters) iters[v] = fo.iters;                                                                         \
@@ -1350,8 +1410,8 @@ typedef struct orc_samples {
   }                                                                                                           \
   int PFX##_mcmc(const orc_opts* o, int N, const double* bvals, const double* bvecs, int nvox,                \
                  const double* data, const double* params, const double* tau, uint64_t vox_offset,            \
-                 orc_samples* out, double* final_energy) {                                                    \
-    orc::Acq<RT> acq(N, bvals, bvecs);                                                                         \
+                 orc_samples* out, double* final_energy, const double* grad) {                                \
+    orc::Acq<RT> acq0(N, bvals, bvecs);                                                                        \
     int K = o->nfibres;                                                                                       \
     int npo = 2 + 3 * K + (o->model >= 2 ? 1 : 0) + (o->f0 ? 1 : 0);                                          \
     int ns = o->njumps / o->sampleevery;                                                                      \
@@ -1362,7 +1422,9 @@ typedef struct orc_samples {
       if (o->rician) orc::remove_nonpositive<RT>(y);                                                           \
       std::vector<RT> p(npo);                                                                                  \
       for (int i = 0; i < npo; ++i) p[i] = (RT)params[(size_t)v * npo + i];                                    \
-      orc::Chain<RT> ch(*o, acq, y.data());                                                                    \
+      std::vector<double> none;                                                                                \
+      orc::VoxAcq<RT> va(N, bvals, bvecs, grad ? grad + (size_t)v * 9 : nullptr, acq0, none, false);            \
+      orc::Chain<RT> ch(*o, *va.acq, y.data());                                                                \
       ch.init(p.data(), tau ? (RT)tau[v] : RT(0));                                                              \
       uint64_t gv = vox_offset + (uint64_t)v;                                                                 \
       int total = o->nburn + o->njumps;                                                                       \
@@ -1410,6 +1472,11 @@ int orc_dti(int N, const double* bvals, const double* bvecs, int nvox, const dou
   return 0;
 }
 
+int orc_correct_acq(int N, const double* bvals, const double* bvecs, const double* g, double* bvals_c,
+                    double* bvecs_c) {
+  orc::correct_acq(N, bvals, bvecs, g, bvals_c, bvecs_c);
+  return 0;
+}
 int orc_pinv(int m, int n, const double* A, double* P) { orc::pinv(m, n, A, P); return 0; }
 int orc_eig(int n, const double* A, double* w, double* V) { orc::jacobi_eig(n, A, w, V); return 0; }
 int orc_philox(const uint32_t* ctr, const uint32_t* key, uint32_t* out) { orc::philox4x32_10(ctr, key, out); return 0; }
