@@ -21,7 +21,7 @@ LAYOUT_VOX_MAJOR, LAYOUT_VOL_MAJOR = 0, 1
 KIND_SINGLE, KIND_SINGLE_C, KIND_MULTI = 0, 1, 2
 
 EXPORTS = ["fdt_default_opts", "fdt_create", "fdt_destroy", "fdt_last_error", "fdt_set_acquisition",
-           "fdt_load_data", "fdt_num_params", "fdt_fit", "fdt_lm_stage", "fdt_lm_start", "fdt_get_params",
+           "fdt_load_data", "fdt_load_grad_dev", "fdt_get_voxel_acquisition", "fdt_num_params", "fdt_fit", "fdt_lm_stage", "fdt_lm_start", "fdt_get_params",
            "fdt_set_params", "fdt_mcmc", "fdt_finish", "fdt_get_outputs", "fdt_run", "fdt_sync",
            "fdt_get_timing", "fdt_set_mcmc_tile",
            "fdt_queue_create", "fdt_queue_destroy", "fdt_queue_last_error", "fdt_enqueue_batch", "fdt_run_queue",
@@ -81,6 +81,8 @@ def lib():
         L.fdt_last_error.restype = C.c_char_p
         L.fdt_set_acquisition.argtypes = [vp, C.c_int, _FP, _FP]
         L.fdt_load_data.argtypes = [vp, _FP, C.c_int, C.c_int]
+        L.fdt_load_grad_dev.argtypes = [vp, _FP, C.c_int, C.c_int]
+        L.fdt_get_voxel_acquisition.argtypes = [vp, C.c_int, _FP, _FP]
         L.fdt_num_params.argtypes = [vp]
         L.fdt_fit.argtypes = [vp]
         L.fdt_lm_stage.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float, _FP, _FP, ip]
@@ -194,6 +196,23 @@ class Xfibres:
         nvox = d.shape[0] if layout == LAYOUT_VOX_MAJOR else d.shape[1]
         self._chk(self.L.fdt_load_data(self.h, dp, layout, nvox))
         self.nvox = nvox
+
+    def load_grad_dev(self, grad, layout=LAYOUT_VOX_MAJOR):
+        """--gradnonlin: per-voxel gradient deviations, [nvox][9] (or [9][nvox]
+        with LAYOUT_VOL_MAJOR); None returns to the shared acquisition"""
+        if grad is None:
+            self._chk(self.L.fdt_load_grad_dev(self.h, _FP(), 0, 0))
+            return
+        g, gp = _f32(grad)
+        nvox = g.shape[0] if layout == LAYOUT_VOX_MAJOR else g.shape[1]
+        self._chk(self.L.fdt_load_grad_dev(self.h, gp, layout, nvox))
+
+    def voxel_acquisition(self, vox):
+        """(bvals [N], bvecs [3][N]) that voxel `vox` is fitted with"""
+        b = np.zeros(self.N, np.float32)
+        g = np.zeros((3, self.N), np.float32)
+        self._chk(self.L.fdt_get_voxel_acquisition(self.h, int(vox), b.ctypes.data_as(_FP), g.ctypes.data_as(_FP)))
+        return b, g
 
     @property
     def num_params(self):
@@ -455,12 +474,15 @@ class Queue:
         return arr
 
 
-def run_xfibres(bvals, bvecs, data, seed=8665904, vox_offset=0, layout=LAYOUT_VOX_MAJOR, **opts):
-    """Whole pipeline on one GPU: LM init -> MCMC -> finish; returns outputs and params."""
+def run_xfibres(bvals, bvecs, data, seed=8665904, vox_offset=0, layout=LAYOUT_VOX_MAJOR, grad=None, **opts):
+    """Whole pipeline on one GPU: LM init -> MCMC -> finish; returns outputs and
+    params.  grad: per-voxel gradient deviations [nvox][9] (--gradnonlin)."""
     x = Xfibres(seed=seed, **opts)
     try:
         x.set_acquisition(bvals, bvecs)
         x.load_data(data, layout)
+        if grad is not None:
+            x.load_grad_dev(grad)
         x.fit()
         params, tau = x.get_params()
         x.mcmc(vox_offset)

@@ -47,6 +47,8 @@ hipError_t fdt_launch_sg_decode(const SgDecodeArgs*, hipStream_t);
 hipError_t fdt_launch_sg_fallback(const SgFallbackArgs*, hipStream_t);
 hipError_t fdt_launch_finish(const FinishArgs*, hipStream_t);
 hipError_t fdt_launch_transpose(const float*, float*, int, int, hipStream_t);
+hipError_t fdt_launch_grad_acq(const float4* base, int N, int stride, const float* grad, int nvox, float4* out,
+                               hipStream_t s);
 #define FDT_VPL_DECL(V) hipError_t fdt_launch_mcmc_vpl##V(const McmcArgs*, int, int, hipStream_t);
 FDT_VPL_DECL(3)
 FDT_VPL_DECL(5)
@@ -62,6 +64,13 @@ typedef hipError_t (*mcmc_launch_fn)(const McmcArgs*, int, int, hipStream_t);
 struct VplEntry { int vpl; mcmc_launch_fn fn; };
 const VplEntry kVpl[] = {{3, fdt_launch_mcmc_vpl3}, {5, fdt_launch_mcmc_vpl5}, {10, fdt_launch_mcmc_vpl10},
                          {18, fdt_launch_mcmc_vpl18}, {24, fdt_launch_mcmc_vpl24}, {32, fdt_launch_mcmc_vpl32}};
+
+// the MCMC tile (volumes per lane) for N volumes, at least `force`
+int tile_of(int N, int force) {
+  for (const auto& e : kVpl)
+    if (16 * e.vpl >= N && e.vpl >= force) return e.vpl;
+  return 0;
+}
 
 template <class T>
 struct DevBuf {
@@ -185,6 +194,12 @@ struct fdt_ctx {
   bool quiet = false;  // no timing events (launches recorded into a graph)
   uint64_t vox_offset = 0;
   hipEvent_t ev_done = nullptr;
+  // --gradnonlin: per-voxel gradient deviations [grad_nvox][9] and the corrected
+  // acquisitions built from them [nvox][acqv_stride] (rebuilt when stale)
+  bool gn = false, acqv_valid = false;
+  int grad_nvox = 0, acqv_stride = 0;
+  DevBuf<float> d_grad;
+  DevBuf<float4> d_acqv;
 };
 
 static int set_err(fdt_ctx* c, int code, const char* fmt, ...) {
@@ -214,6 +229,8 @@ static int check_opts(fdt_ctx* c, const fdt_opts& o) {
   if (o.njumps > 0 && nsamples_of(o) == 0) return set_err(c, FDT_EINVAL, "njumps < sampleevery gives no samples");
   return FDT_OK;
 }
+
+static int ensure_acqv(fdt_ctx* c);
 
 extern "C" {
 
@@ -256,6 +273,8 @@ void fdt_destroy(fdt_ctx* c) {
   for (auto* b : fb) b->release();
   c->d_pinv.release();
   c->d_acq.release();
+  c->d_grad.release();
+  c->d_acqv.release();
   c->d_shell.release();
   c->d_shell_b.release();
   c->d_iters.release();
@@ -315,6 +334,7 @@ int fdt_set_acquisition(fdt_ctx* c, int N, const float* bvals, const float* bvec
   HIPCHK(c, hipMemcpyAsync(c->d_shell_b.p, shb.data(), sizeof(float) * 16, hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->nvox = 0;
+  c->acqv_valid = false;
   c->have_params = c->have_samples = c->have_finish = false;
   return FDT_OK;
 }
@@ -339,7 +359,61 @@ int fdt_load_data(fdt_ctx* c, const float* data, int layout, int nvox) {
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->d_stage.release();
   c->nvox = nvox;
+  c->acqv_valid = false;
   c->have_params = c->have_samples = c->have_finish = false;
+  return FDT_OK;
+}
+
+int fdt_load_grad_dev(fdt_ctx* c, const float* grad, int layout, int nvox) {
+  if (!c) return FDT_EINVAL;
+  if (!grad) {  // back to the shared acquisition
+    c->gn = c->acqv_valid = false;
+    c->grad_nvox = 0;
+    c->d_grad.release();
+    c->d_acqv.release();
+    c->have_params = c->have_samples = c->have_finish = false;
+    return FDT_OK;
+  }
+  if (nvox <= 0) return set_err(c, FDT_EINVAL, "nvox=%d", nvox);
+  if (layout != FDT_LAYOUT_VOX_MAJOR && layout != FDT_LAYOUT_VOL_MAJOR) return set_err(c, FDT_EINVAL, "bad layout %d", layout);
+  (void)hipSetDevice(c->device);
+  size_t n = (size_t)nvox * 9;
+  HIPCHK(c, c->d_grad.ensure(n));
+  if (layout == FDT_LAYOUT_VOX_MAJOR) {
+    HIPCHK(c, hipMemcpyAsync(c->d_grad.p, grad, n * sizeof(float), hipMemcpyHostToDevice, c->stream));
+  } else {
+    HIPCHK(c, c->d_stage.ensure(n));
+    HIPCHK(c, hipMemcpyAsync(c->d_stage.p, grad, n * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, fdt_launch_transpose(c->d_stage.p, c->d_grad.p, 9, nvox, c->stream));
+  }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->d_stage.release();
+  c->gn = true;
+  c->acqv_valid = false;
+  c->grad_nvox = nvox;
+  c->have_params = c->have_samples = c->have_finish = false;
+  return FDT_OK;
+}
+
+int fdt_get_voxel_acquisition(fdt_ctx* c, int vox, float* bvals, float* bvecs) {
+  if (!c || !bvals || !bvecs) return set_err(c, FDT_EINVAL, "null argument");
+  if (vox < 0 || vox >= c->nvox) return set_err(c, FDT_EINVAL, "voxel %d of %d", vox, c->nvox);
+  (void)hipSetDevice(c->device);
+  const int N = c->N;
+  if (!c->gn) {
+    std::copy(c->bvals.begin(), c->bvals.end(), bvals);
+    std::copy(c->bvecs.begin(), c->bvecs.end(), bvecs);
+    return FDT_OK;
+  }
+  if (int rc = ensure_acqv(c)) return rc;
+  std::vector<float4> row(N);
+  HIPCHK(c, hipMemcpyAsync(row.data(), c->d_acqv.p + (size_t)vox * c->acqv_stride, N * sizeof(float4),
+                           hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (int i = 0; i < N; ++i) {
+    bvals[i] = row[i].w;
+    bvecs[i] = row[i].x; bvecs[N + i] = row[i].y; bvecs[2 * N + i] = row[i].z;
+  }
   return FDT_OK;
 }
 
@@ -369,6 +443,22 @@ static hipError_t launch_tau(const TauArgs& a, int K, int fp64, hipStream_t s) {
   return hipErrorInvalidValue;
 }
 
+// --gradnonlin: (re)build the per-voxel acquisition table for the current
+// acquisition, deviations and MCMC tile (stride = 16 x the tile's volumes per lane)
+static int ensure_acqv(fdt_ctx* c) {
+  if (!c->gn) return FDT_OK;
+  if (c->grad_nvox != c->nvox)
+    return set_err(c, FDT_ESTATE, "gradient deviations for %d voxels, data for %d", c->grad_nvox, c->nvox);
+  int stride = 16 * tile_of(c->N, c->vpl_force);
+  if (stride <= 0) return set_err(c, FDT_EUNSUP, "no MCMC tile for N=%d", c->N);
+  if (c->acqv_valid && stride == c->acqv_stride) return FDT_OK;
+  HIPCHK(c, c->d_acqv.ensure((size_t)std::max(c->nvox, 1) * stride));
+  HIPCHK(c, fdt_launch_grad_acq(c->d_acq.p, c->N, stride, c->d_grad.p, c->nvox, c->d_acqv.p, c->stream));
+  c->acqv_stride = stride;
+  c->acqv_valid = true;
+  return FDT_OK;
+}
+
 static LmArgs base_lm_args(fdt_ctx* c) {
   LmArgs a{};
   a.data = c->d_data.p;
@@ -378,6 +468,8 @@ static LmArgs base_lm_args(fdt_ctx* c) {
   a.pinvA = c->d_pinv.p;
   a.lm = 1;
   a.R = c->o.R_prior_mean;
+  a.acqv = c->gn ? c->d_acqv.p : nullptr;
+  a.acqv_stride = c->acqv_stride;
   return a;
 }
 
@@ -402,6 +494,7 @@ static int fit_async(fdt_ctx* c) {
     HIPCHK(c, c->d_refit.ensure(nv));
   }
   int* q = c->d_queue.p;
+  if (int rc = ensure_acqv(c)) return rc;
   const int k1 = o.model == 1 ? (o.nonlin ? KIND_SINGLE : KIND_SINGLE_C) : KIND_SINGLE;
   const int G = o.model == 3 ? 2 : 0;
   if (!c->quiet) HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
@@ -478,6 +571,8 @@ static int fit_async(fdt_ctx* c) {
     t.R = o.R_prior_mean;
     t.refit = f0 ? c->d_refit.p : nullptr;
     t.tau = c->d_tau.p;
+    t.acqv = c->gn ? c->d_acqv.p : nullptr;
+    t.acqv_stride = c->acqv_stride;
     HIPCHK(c, launch_tau(t, K, fp64, c->stream));
   }
   if (!c->quiet) HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
@@ -509,6 +604,7 @@ int fdt_lm_stage(fdt_ctx* c, int kind, int K, int f0, int G, float R, const floa
   HIPCHK(c, dout.ensure(n));
   HIPCHK(c, dit.ensure(c->nvox));
   HIPCHK(c, hipMemcpyAsync(ds.p, start, n * sizeof(float), hipMemcpyHostToDevice, c->stream));
+  if (int rc = ensure_acqv(c)) return rc;
   HIPCHK(c, c->d_queue.ensure(NQUEUE));
   HIPCHK(c, fdt_launch_zero_ints(c->d_queue.p, NQUEUE, c->stream));
   LmArgs a = base_lm_args(c);
@@ -538,6 +634,7 @@ int fdt_lm_start(fdt_ctx* c, int kind, int K, int f0, float* out) {
   size_t n = (size_t)c->nvox * np;
   DevBuf<float> dout;
   HIPCHK(c, dout.ensure(n));
+  if (int rc = ensure_acqv(c)) return rc;
   HIPCHK(c, c->d_queue.ensure(NQUEUE));
   HIPCHK(c, fdt_launch_zero_ints(c->d_queue.p, NQUEUE, c->stream));
   LmArgs a = base_lm_args(c);
@@ -606,6 +703,7 @@ static int mcmc_prepare(fdt_ctx* c, uint64_t vox_offset, McmcArgs* out, mcmc_lau
     if (16 * e.vpl >= c->N && e.vpl >= want) { vpl = e.vpl; fn = e.fn; break; }
   if (!fn) return set_err(c, FDT_EUNSUP, "no MCMC tile for N=%d", c->N);
   c->vpl_used = vpl;
+  if (int rc = ensure_acqv(c)) return rc;
   McmcArgs a{};
   a.data = c->d_data.p;
   a.params = c->d_params.p;
@@ -645,6 +743,11 @@ static int mcmc_prepare(fdt_ctx* c, uint64_t vox_offset, McmcArgs* out, mcmc_lau
   HIPCHK(c, c->d_widths_out.ensure(nv * 16));
   a.widths_out = c->d_widths_out.p;
   a.widths_in = c->use_widths_in ? c->d_widths_in.p : nullptr;
+  if (c->gn) {  // per-voxel b values: no shared b-shell table
+    a.acqv = c->d_acqv.p;
+    a.acqv_stride = c->acqv_stride;
+    a.nshell = 0;
+  }
   *out = a;
   *fn_out = fn;
   return FDT_OK;
@@ -791,12 +894,6 @@ int fdt_set_mcmc_tile(fdt_ctx* c, int vpl) {
 // finishing passes.  The first fdt_run_queue runs eagerly (allocating and
 // uploading everything) and records the same launches into a hipGraph that
 // later runs replay.
-
-static int tile_of(int N, int force) {
-  for (const auto& e : kVpl)
-    if (16 * e.vpl >= N && e.vpl >= force) return e.vpl;
-  return 0;
-}
 
 }  // extern "C"
 
@@ -1463,6 +1560,7 @@ int fdt_run_surrogate(fdt_ctx* c, fdt_surrogate* s, uint64_t vox_offset, int nbu
                       float ang_max_deg, int* n_fallback) {
   if (!c || !s) return FDT_EINVAL;
   if (nburn_short < 0) return set_err(c, FDT_EINVAL, "nburn_short=%d", nburn_short);
+  if (c->gn) return set_err(c, FDT_EUNSUP, "the learned start has no per-voxel acquisitions (--gradnonlin)");
   (void)hipSetDevice(c->device);
   const fdt_opts& o = c->o;
   const int nv = c->nvox, K = o.nfibres;

@@ -577,13 +577,16 @@ FDT_DEV void stage_acq(const McmcArgs& a, const ChainLds& L) {
 // The whole chain of voxel `vox` on this thread's 16-lane row: initialisation
 // (init_Fibres_Multifibres_kernel, CUDA/runmcmc_kernels.cu:193-505), nburn +
 // njumps sweeps with recording and proposal adaptation (runmcmc_kernel,
-// :507-1474, run twice by CUDA/runmcmc.cu:108-316).
-template <int K, int VPL, bool RIC>
+// :507-1474, run twice by CUDA/runmcmc.cu:108-316).  GN (--gradnonlin): the
+// chain reads its voxel's own corrected acquisition from a.acqv (the reference
+// indexes per-voxel bvals/alpha/beta with posBV, :266, :711).
+template <int K, int VPL, bool RIC, bool GN = false>
 FDT_DEV void run_chain(const McmcArgs& a, int vox, const ChainLds& L) {
   using C = Chain<K, VPL, RIC>;
   const int lane = __lane_id();
   const int rl = lane & 15;
   const float4* sacq = L.acq;
+  if constexpr (GN) sacq = a.acqv + (size_t)vox * a.acqv_stride;
   C c;
   c.rl = rl;
   c.acq = sacq;

@@ -29,7 +29,7 @@ namespace fdt {
 struct Row {
   int rl;
   int N;
-  const float4* acq;  // LDS, [N]
+  const float4* acq;  // [N]: the workgroup's table in LDS, or the voxel's own (--gradnonlin)
   const float* y;     // LDS, the voxel's signal (sanitised for --rician)
 };
 
@@ -617,6 +617,8 @@ FDT_DEV void cart2sph(const double v[3], double& th, double& ph) {
   th = atan2(sqrt(v[0] * v[0] + v[1] * v[1]), v[2]);
 }
 
+FDT_DEV Dti tensor_of(double s0, const double (&D)[7]);
+
 // DTI::linfit (diffmodels.cc:17-46) + calc_tensor_parameters (:239-265)
 FDT_DEV Dti dti_row(const Row& R, const double* pinvA) {
   const int N = R.N;
@@ -651,6 +653,11 @@ FDT_DEV Dti dti_row(const Row& R, const double* pinvA) {
   for (int j = 0; j < 7; ++j) D[j] = -row_sum16_d(D[j]);
   s0 = exp(-D[6]);
   if (s0 < mean) s0 = mean;
+  return tensor_of(s0, D);
+}
+
+// calc_tensor_parameters (diffmodels.cc:239-265) of the tensor D
+FDT_DEV Dti tensor_of(double s0, const double (&D)[7]) {
   Dti o;
   o.s0 = s0;
   double Tn[9] = {D[0], D[1], D[2], D[1], D[3], D[4], D[2], D[4], D[5]};
@@ -664,6 +671,99 @@ FDT_DEV Dti dti_row(const Row& R, const double* pinvA) {
 }
 
 // PVM_single start values (CUDA/diffmodels.cu:72-93)
+// DTI::linfit of a voxel with its own acquisition (--gradnonlin: the
+// reference forms and pseudo-inverts the design matrix per voxel,
+// CUDA/diffmodels.cu:57-63): the same two log-linear passes as dti_row, each
+// solved through the column-equilibrated normal equations (G = As^T As, row a
+// of G in lane a, LU in FP64); equal to the pseudo-inverse for a full-rank design.
+FDT_DEV void design_row(float4 q, double (&r)[7]) {
+  double x = q.x, y = q.y, z = q.z, b = q.w;
+  r[0] = x * x * b; r[1] = 2 * x * y * b; r[2] = 2 * x * z * b;
+  r[3] = y * y * b; r[4] = 2 * y * z * b; r[5] = z * z * b; r[6] = 1.0;
+}
+
+// D = -pinv(A) l for l_i = lval(y_i)
+template <class F>
+FDT_DEV void lls_row(const Row& R, const double (&Grow)[7], const double (&sc)[7], F&& lval, double (&D)[7]) {
+  double h[7];
+#pragma unroll
+  for (int j = 0; j < 7; ++j) h[j] = 0.0;
+  for (int i = R.rl; i < R.N; i += 16) {
+    double r[7];
+    design_row(R.acq[i], r);
+    const double l = lval(R.y[i]);
+#pragma unroll
+    for (int j = 0; j < 7; ++j) h[j] += r[j] / sc[j] * l;
+  }
+  double hr = 0.0;
+#pragma unroll
+  for (int j = 0; j < 7; ++j) {
+    h[j] = row_sum16_d(h[j]);
+    if (R.rl == j) hr = h[j];
+  }
+  double x[7];
+  lu_row<7>(Grow, hr, x, R.rl);
+#pragma unroll
+  for (int j = 0; j < 7; ++j) D[j] = -x[j] / sc[j];
+}
+
+FDT_DEV Dti dti_row_gn(const Row& R) {
+  const int N = R.N;
+  double ysum = 0;
+  float ymx = 0.0f;
+  for (int i = R.rl; i < N; i += 16) { float y = R.y[i]; ysum += (double)y; ymx = fmaxf(ymx, fabsf(y)); }
+  ysum = row_sum16_d(ysum);
+  double ymax = (double)row_max16(ymx);
+  double mean = ysum / N;
+  // column scales and the equilibrated normal matrix
+  double sc[7], G[28];
+#pragma unroll
+  for (int j = 0; j < 7; ++j) sc[j] = 0.0;
+  for (int i = R.rl; i < N; i += 16) {
+    double r[7];
+    design_row(R.acq[i], r);
+#pragma unroll
+    for (int j = 0; j < 7; ++j) sc[j] += r[j] * r[j];
+  }
+#pragma unroll
+  for (int j = 0; j < 7; ++j) { double v = row_sum16_d(sc[j]); sc[j] = v > 0 ? sqrt(v) : 1.0; }
+#pragma unroll
+  for (int e = 0; e < 28; ++e) G[e] = 0.0;
+  for (int i = R.rl; i < N; i += 16) {
+    double r[7];
+    design_row(R.acq[i], r);
+#pragma unroll
+    for (int j = 0; j < 7; ++j) r[j] /= sc[j];
+    int e = 0;
+#pragma unroll
+    for (int a = 0; a < 7; ++a)
+#pragma unroll
+      for (int c = a; c < 7; ++c) G[e++] += r[a] * r[c];
+  }
+  double Grow[7];
+#pragma unroll
+  for (int c = 0; c < 7; ++c) Grow[c] = 0.0;
+  {
+    int e = 0;
+#pragma unroll
+    for (int a = 0; a < 7; ++a)
+#pragma unroll
+      for (int c = a; c < 7; ++c) {
+        double v = row_sum16_d(G[e++]);
+        if (R.rl == a) Grow[c] = v;
+        if (R.rl == c) Grow[a] = v;
+      }
+  }
+  double D[7];
+  lls_row(R, Grow, sc, [](float y) { return y > 0.0f ? log((double)y) : 0.0; }, D);
+  double s0 = (D[6] > -23) ? exp(-D[6]) : ymax;
+  if (s0 < mean) s0 = ymax;
+  lls_row(R, Grow, sc, [s0](float yf) { double y = (double)yf; return (y / s0) > 0.01 ? log(y) : log(0.01 * s0); }, D);
+  s0 = exp(-D[6]);
+  if (s0 < mean) s0 = mean;
+  return tensor_of(s0, D);
+}
+
 template <class T, int K, bool F0>
 FDT_DEV void start_single(const Dti& dti, T* p) {
   constexpr int NP = 2 + 3 * K + (F0 ? 1 : 0);
@@ -836,12 +936,15 @@ FDT_DEV void store_vec(float* dst, const T (&v)[NP], int rl) {
     if (rl == j) dst[j] = (float)v[j];
 }
 
-template <class T, int KIND, int K, bool F0>
+// GN: --gradnonlin, every voxel reads its own corrected acquisition from
+// a.acqv instead of the workgroup's table in LDS.
+template <class T, int KIND, int K, bool F0, bool GN>
 __global__ __launch_bounds__(256) void lm_kernel(LmArgs a) {
   extern __shared__ float4 lds4[];
   float4* sacq = lds4;
   float* sy = reinterpret_cast<float*>(lds4 + a.N);
-  for (int i = threadIdx.x; i < a.N; i += blockDim.x) sacq[i] = a.acq[i];
+  if constexpr (!GN)
+    for (int i = threadIdx.x; i < a.N; i += blockDim.x) sacq[i] = a.acq[i];
   __syncthreads();
   using M = Mdl<T, KIND, K, F0>;
   constexpr int NP = M::NP;
@@ -864,6 +967,7 @@ __global__ __launch_bounds__(256) void lm_kernel(LmArgs a) {
       q = row_bcast_i<0>(q);
       if (q >= count) break;
       vox = a.list ? a.list[q] : q;
+      if constexpr (GN) R.acq = a.acqv + (size_t)vox * a.acqv_stride;
       // the voxel's signal row, non-positive samples replaced by the smallest
       // positive one for --rician (remove_NonPositive_entries, CUDA/xfibres_gpu.cu:350-367)
       const float* src = a.data + (size_t)vox * a.N;
@@ -888,7 +992,9 @@ __global__ __launch_bounds__(256) void lm_kernel(LmArgs a) {
         for (int i = 0; i < NP - 1; ++i) s1[i] = (T)a.in[(size_t)vox * a.in_np + i];
         start_multi<T, K, F0>(s1, S.p);
       } else {
-        Dti dti = dti_row(R, a.pinvA);
+        Dti dti;
+        if constexpr (GN) dti = dti_row_gn(R);
+        else dti = dti_row(R, a.pinvA);
         if constexpr (KIND == KIND_SINGLE) start_single<T, K, F0>(dti, S.p);
         else start_single_c<T, K, F0>(dti, R, S.p);
       }
@@ -919,6 +1025,7 @@ __global__ __launch_bounds__(256) void tau_kernel(TauArgs a) {
   const int vox = blockIdx.x * 16 + (threadIdx.x >> 4);
   if (vox >= a.nvox) return;
   const float* y = a.data + (size_t)vox * a.N;
+  const float4* acq = a.acqv ? a.acqv + (size_t)vox * a.acqv_stride : sacq;
   float mn = 3.0e38f, mx = -3.0e38f, mnp = 3.0e38f;
   for (int i = rl; i < a.N; i += 16) {
     float v = y[i];
@@ -950,7 +1057,7 @@ __global__ __launch_bounds__(256) void tau_kernel(TauArgs a) {
   for (int pass = 0; pass < 2; ++pass) {
     double acc = 0.0;
     for (int i = rl; i < a.N; i += 16) {
-      float4 g = sacq[i];
+      float4 g = acq[i];
       T b = (T)g.w;
       T iso = model >= 2 ? exp(-a_ * log(T(1) + b * b_)) : exp(-b * dd);
       T an = T(0);
@@ -991,9 +1098,9 @@ using FitReal = std::conditional<FDT_FIT_FP64 != 0, double, float>::type;
 
 namespace {
 // persistent grid: as many workgroups as stay resident, at most one row per voxel
-template <int KIND, bool F0>
+template <int KIND, bool F0, bool GN>
 hipError_t launch_lm(const fdt::LmArgs* a, hipStream_t s) {
-  auto kern = fdt::lm_kernel<FitReal, KIND, FDT_FIT_K, F0>;
+  auto kern = fdt::lm_kernel<FitReal, KIND, FDT_FIT_K, F0, GN>;
   size_t lds = (size_t)a->N * sizeof(float4) + (size_t)fdt::LM_ROWS * (16 * (((a->N + 15) / 16) | 1)) * sizeof(float);
   int dev = 0, cus = 0, per_cu = 0;
   hipError_t e;
@@ -1006,16 +1113,23 @@ hipError_t launch_lm(const fdt::LmArgs* a, hipStream_t s) {
   hipLaunchKernelGGL(kern, dim3(grid), dim3(256), lds, s, *a);
   return hipGetLastError();
 }
+
+template <bool GN>
+hipError_t launch_kind(const fdt::LmArgs* a, int kind, int f0, hipStream_t s) {
+  using namespace fdt;
+  if (kind == KIND_SINGLE) return f0 ? launch_lm<KIND_SINGLE, true, GN>(a, s) : launch_lm<KIND_SINGLE, false, GN>(a, s);
+  if (kind == KIND_SINGLE_C) return f0 ? launch_lm<KIND_SINGLE_C, true, GN>(a, s) : launch_lm<KIND_SINGLE_C, false, GN>(a, s);
+  if (kind == KIND_MULTI) return f0 ? launch_lm<KIND_MULTI, true, GN>(a, s) : launch_lm<KIND_MULTI, false, GN>(a, s);
+  return hipErrorInvalidValue;
+}
 }  // namespace
 
 extern "C" hipError_t FDT_CAT3(fdt_launch_lm_k, FDT_FIT_K, FDT_FIT_FP64)(const fdt::LmArgs* a, int kind, int f0,
                                                                        hipStream_t s) {
   if (a->nvox <= 0) return hipSuccess;
   if (a->N > fdt::MAX_N) return hipErrorInvalidValue;
-  if (kind == fdt::KIND_SINGLE) return f0 ? launch_lm<fdt::KIND_SINGLE, true>(a, s) : launch_lm<fdt::KIND_SINGLE, false>(a, s);
-  if (kind == fdt::KIND_SINGLE_C) return f0 ? launch_lm<fdt::KIND_SINGLE_C, true>(a, s) : launch_lm<fdt::KIND_SINGLE_C, false>(a, s);
-  if (kind == fdt::KIND_MULTI) return f0 ? launch_lm<fdt::KIND_MULTI, true>(a, s) : launch_lm<fdt::KIND_MULTI, false>(a, s);
-  return hipErrorInvalidValue;
+  if (a->acqv) return launch_kind<true>(a, kind, f0, s);
+  return launch_kind<false>(a, kind, f0, s);
 }
 
 extern "C" hipError_t FDT_CAT3(fdt_launch_tau_k, FDT_FIT_K, FDT_FIT_FP64)(const fdt::TauArgs* a, hipStream_t s) {

@@ -36,6 +36,8 @@ struct LmArgs {
   int set_f0_slot;      // also write 0.001 into the last slot (refit without f0)
   int* iters;           // optional [nvox]: LM iterations
   int iters_add;        // add to iters instead of overwriting
+  const float4* acqv;   // --gradnonlin: per-voxel acquisitions [nvox][acqv_stride], or null
+  int acqv_stride;
 };
 
 // implausible-f0 test and refit list (CUDA/xfibres_gpu.cu:216-246, 305-339)
@@ -58,6 +60,8 @@ struct TauArgs {
   float R;
   const unsigned char* refit;  // voxels refitted without f0, or null
   float* tau;
+  const float4* acqv;   // --gradnonlin: per-voxel acquisitions, or null
+  int acqv_stride;
 };
 
 struct McmcArgs {
@@ -80,6 +84,8 @@ struct McmcArgs {
   int vstride;          // voxel count of the sample arrays (>= every voxel index)
   const float* widths_in;  // [vstride][16] initial proposal width per slot (<= 0: default), optional
   float* widths_out;    // [vstride][16] final proposal widths, optional
+  const float4* acqv;   // --gradnonlin: per-voxel acquisitions [nvox][acqv_stride >= Npad], or null
+  int acqv_stride;
 };
 
 // Batched MCMC over a queue of volumes (fdt_queue.hip): work items are runs

@@ -5,7 +5,7 @@
 
 namespace fdt {
 
-template <int K, int VPL, bool RIC>
+template <int K, int VPL, bool RIC, bool GN>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VPL <= 18 ? 2 : 1, 2))) void mcmc_kernel(McmcArgs a) {
   __shared__ float4 sacq[16 * VPL];
   __shared__ float sba[4 * K * VPL * 64];
@@ -16,13 +16,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VPL <= 18 ?
   stage_acq<VPL>(a, L);
   __syncthreads();
   const int i = blockIdx.x * 16 + (threadIdx.x >> 4);
-  if (i < a.nvox) run_chain<K, VPL, RIC>(a, a.list ? a.list[i] : i, L);
+  if (i < a.nvox) run_chain<K, VPL, RIC, GN>(a, a.list ? a.list[i] : i, L);
 }
 
 }  // namespace fdt
 
 // One translation unit per VPL value (-DFDT_VPL=...), instantiating K=1..3
-// for Gaussian and Rician noise.
+// for Gaussian and Rician noise, with the shared acquisition or per-voxel
+// ones (--gradnonlin, a->acqv set; its stride must cover 16 * VPL volumes).
 #ifndef FDT_VPL
 #define FDT_VPL 18
 #endif
@@ -30,11 +31,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VPL <= 18 ?
 #define FDT_CAT2(a, b) FDT_CAT2_(a, b)
 extern "C" hipError_t FDT_CAT2(fdt_launch_mcmc_vpl, FDT_VPL)(const fdt::McmcArgs* a, int K, int rician, hipStream_t s) {
   if (a->nvox <= 0) return hipSuccess;
+  if (a->acqv && a->acqv_stride < 16 * FDT_VPL) return hipErrorInvalidValue;
   dim3 grid((a->nvox + 15) / 16), block(256);
-#define FDT_MC_CASE(KK, RR) \
-  if (K == KK && (rician != 0) == RR) { hipLaunchKernelGGL((fdt::mcmc_kernel<KK, FDT_VPL, RR>), grid, block, 0, s, *a); return hipGetLastError(); }
-  FDT_MC_CASE(1, false) FDT_MC_CASE(2, false) FDT_MC_CASE(3, false)
-  FDT_MC_CASE(1, true) FDT_MC_CASE(2, true) FDT_MC_CASE(3, true)
+  const bool gn = a->acqv != nullptr;
+#define FDT_MC_CASE(KK, RR, GG) \
+  if (K == KK && (rician != 0) == RR && gn == GG) { hipLaunchKernelGGL((fdt::mcmc_kernel<KK, FDT_VPL, RR, GG>), grid, block, 0, s, *a); return hipGetLastError(); }
+  FDT_MC_CASE(1, false, false) FDT_MC_CASE(2, false, false) FDT_MC_CASE(3, false, false)
+  FDT_MC_CASE(1, true, false) FDT_MC_CASE(2, true, false) FDT_MC_CASE(3, true, false)
+  FDT_MC_CASE(1, false, true) FDT_MC_CASE(2, false, true) FDT_MC_CASE(3, false, true)
+  FDT_MC_CASE(1, true, true) FDT_MC_CASE(2, true, true) FDT_MC_CASE(3, true, true)
 #undef FDT_MC_CASE
   return hipErrorInvalidValue;
 }

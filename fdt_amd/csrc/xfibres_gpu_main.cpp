@@ -263,27 +263,28 @@ int main(int argc, char* argv[]) {
     std::cerr << usage_tail << std::endl << "\nNumThisPart: " << idPart << " is outside 0.." << nParts - 1 << std::endl;
     return EXIT_FAILURE;
   }
-  if (c.grad_set) {
-    std::cerr << "xfibres_gpu: --gradnonlin (per-voxel acquisitions) is not supported by this build" << std::endl;
-    return EXIT_FAILURE;
-  }
-
   // this part of the data (xfibres_gpu.cc:104-113)
   int size_part = totalNvox / nParts;
   uint64_t vox_offset = (uint64_t)idPart * (uint64_t)size_part;
   if (idPart == nParts - 1) size_part = totalNvox - size_part * (nParts - 1);
-  std::vector<float> data((size_t)N * size_part);
-  {
-    std::ifstream in(c.data, std::ios::binary);
-    if (!in) { std::cerr << "cannot open data file " << c.data << std::endl; return EXIT_FAILURE; }
-    if (real_bytes == 4) in.read(reinterpret_cast<char*>(data.data()), data.size() * sizeof(float));
+  // raw `Real` [rows][size_part] files: the data part and the grad_dev part
+  // (xfibres_gpu.cc:108-122; the gradient deviations are 9 rows)
+  auto read_real = [&](const std::string& path, const char* what, int rows, std::vector<float>& out) {
+    out.assign((size_t)rows * size_part, 0.0f);
+    std::ifstream in(path, std::ios::binary);
+    if (!in) { std::cerr << "cannot open " << what << " file " << path << std::endl; return false; }
+    if (real_bytes == 4) in.read(reinterpret_cast<char*>(out.data()), out.size() * sizeof(float));
     else {
-      std::vector<double> d(data.size());
+      std::vector<double> d(out.size());
       in.read(reinterpret_cast<char*>(d.data()), d.size() * sizeof(double));
-      for (size_t i = 0; i < d.size(); ++i) data[i] = (float)d[i];
+      for (size_t i = 0; i < d.size(); ++i) out[i] = (float)d[i];
     }
-    if (!in) { std::cerr << "data file " << c.data << " holds fewer than " << N << " x " << size_part << " values" << std::endl; return EXIT_FAILURE; }
-  }
+    if (!in) { std::cerr << what << " file " << path << " holds fewer than " << rows << " x " << size_part << " values" << std::endl; return false; }
+    return true;
+  };
+  std::vector<float> data, grad;
+  if (!read_real(c.data, "data", N, data)) return EXIT_FAILURE;
+  if (c.grad_set && !read_real(c.gradnonlin, "gradnonlin", 9, grad)) return EXIT_FAILURE;
   std::cout << "Number of Voxels to compute in this part: " << size_part << std::endl;
   std::cout << "Number of Directions: " << N << std::endl;
   if (c.o.rician && !c.o.nonlin)
@@ -299,6 +300,7 @@ int main(int argc, char* argv[]) {
   std::cout << "SubPart 1 of 1: processing " << size_part << " voxels" << std::endl;
   if (fdt_set_acquisition(ctx, N, fbvals.data(), fbvecs.data())) return fail("acquisition");
   if (fdt_load_data(ctx, data.data(), FDT_LAYOUT_VOL_MAJOR, size_part)) return fail("data");
+  if (c.grad_set && fdt_load_grad_dev(ctx, grad.data(), FDT_LAYOUT_VOL_MAJOR, size_part)) return fail("gradnonlin");
   if (fdt_run(ctx, vox_offset) || fdt_sync(ctx)) return fail("run");
 
   const int K = c.o.nfibres, ns = c.o.njumps / c.o.sampleevery;

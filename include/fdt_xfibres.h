@@ -121,6 +121,20 @@ int fdt_set_acquisition(fdt_ctx* c, int N, const float* bvals, const float* bvec
  * `layout` is FDT_LAYOUT_VOL_MAJOR for the data_<part> file order. */
 int fdt_load_data(fdt_ctx* c, const float* data, int layout, int nvox);
 
+/* --gradnonlin: the gradient-deviation tensor of every voxel, 9 values
+ * (L(i,j) = g[3j+i]) per voxel; FDT_LAYOUT_VOL_MAJOR is the [9][nvox]
+ * grad_dev_<part> file order (replaces gradm, CUDA/xfibres_gpu.h:19, read by
+ * xfibres_gpu.cc:115-122).  Every later fit and chain then uses each voxel's
+ * corrected bvecs/bvals (correct_bvals_bvecs, CUDA/xfibres_gpu.cu:151-170,
+ * per-voxel acquisitions :403-436), built on the device.  nvox must equal the
+ * loaded data's.  grad == NULL returns to the shared acquisition. */
+int fdt_load_grad_dev(fdt_ctx* c, const float* grad, int layout, int nvox);
+
+/* The acquisition voxel `vox` is fitted with: bvals [N], bvecs 3 x N (the
+ * shared one, or its corrected copy under --gradnonlin).  No counterpart in
+ * the reference; for inspection and tests. */
+int fdt_get_voxel_acquisition(fdt_ctx* c, int vox, float* bvals, float* bvecs);
+
 /* Number of LM output parameters per voxel: 2 + 3K (+1 model>=2) (+1 --f0);
  * layout [S0, d, (d_std), f_k, th_k, ph_k ..., (f0)] (CUDA/xfibres_gpu.cu:450-454). */
 int fdt_num_params(const fdt_ctx* c);

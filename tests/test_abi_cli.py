@@ -102,3 +102,15 @@ def test_cli_logdir_plus_suffix(subj):
     r = run_cli(["--data=d", "--mask=m", "-b", "bvals", "-r", "bvecs", "--logdir=L", "nope", "0", "1", "10"], subj)
     assert "Log directory is: L+" in r.stdout
     assert (subj / "L+").is_dir()
+
+
+def test_cli_gradnonlin_file_checked_before_gpu_work(subj):
+    # a grad_dev part with fewer than 9 x nvox values is refused before the engine starts
+    import numpy as np
+    np.ones((8, 10)).tofile(subj / "data_0")
+    np.zeros((9, 3)).tofile(subj / "grad_short")
+    base = ["--data=data_0", "--mask=m", "-b", "bvals", "-r", "bvecs", "--logdir=L", "--forcedir"]
+    r = run_cli(base + ["--gradnonlin=grad_short", "S", "0", "1", "10"], subj)
+    assert r.returncode == 1 and "gradnonlin file grad_short holds fewer than 9 x 10" in r.stderr
+    r = run_cli(base + ["--gradnonlin=missing", "S", "0", "1", "10"], subj)
+    assert r.returncode == 1 and "cannot open gradnonlin file missing" in r.stderr
