@@ -746,6 +746,7 @@ static int mcmc_prepare(fdt_ctx* c, uint64_t vox_offset, McmcArgs* out, mcmc_lau
   if (c->gn) {  // per-voxel b values: no shared b-shell table
     a.acqv = c->d_acqv.p;
     a.acqv_stride = c->acqv_stride;
+    a.gdev = c->d_grad.p;
     a.nshell = 0;
   }
   *out = a;

@@ -126,7 +126,12 @@ FDT_DEV void sfor(F&& f) {
 // prior energy -- lives in the lane of that slot (sval/sprop/scnt/sprior) and
 // is read by a row broadcast.  Slots: f0 0, tau 1, d 2, d_std 3, R 4, S0 5,
 // theta_k 6+3k, phi_k 7+3k, f_k 8+3k (the theta prior sits in the theta slot).
-template <int K, int VPL, bool RIC>
+// GN (--gradnonlin): the voxel sees gradient g through its deviation M = I+L
+// as direction Mg/|Mg| and b value |Mg|^2 b, so b'(g'.x)^2 = b (g.M^T x)^2:
+// the stick signals use the shared acquisition with the fibre directions
+// mapped by M^T, and the isotropic signal (and the model-3 R term) the
+// voxel's own b values, kept per row in LDS.
+template <int K, int VPL, bool RIC, bool GN = false>
 struct Chain {
   // lane-resident volume state, volume i = rl + 16 t
   float r[VPL];  // Gaussian: y - S0*M; Rician: y
@@ -152,6 +157,23 @@ struct Chain {
   bool lam[K];
   float fudge, halfN, Nf, Rmean, Rstd, Rfudge;
   const float4* acq;
+  const float* bp;  // GN: the voxel's b values, bp[rl + 16 t] (LDS)
+  const float* gd;  // GN: the voxel's 9 deviation values L(i,j) = gd[3j+i] (LDS)
+
+  // b of slot t for the isotropic signal and the model-3 R term
+  FDT_DEV float bv(int t, const float4& g) const {
+    if constexpr (GN) return bp[rl + 16 * t];
+    else return g.w;
+  }
+  // fibre direction as the voxel's gradients see it: x -> M^T x
+  FDT_DEV void xform(float& x0, float& x1, float& x2) const {
+    if constexpr (GN) {
+      float y0 = x0 + fmaf(gd[0], x0, fmaf(gd[1], x1, gd[2] * x2));
+      float y1 = x1 + fmaf(gd[3], x0, fmaf(gd[4], x1, gd[5] * x2));
+      float y2 = x2 + fmaf(gd[6], x0, fmaf(gd[7], x1, gd[8] * x2));
+      x0 = y0; x1 = y1; x2 = y2;
+    }
+  }
 
   static constexpr int TH(int k) { return SL_FIB + 3 * k; }
   static constexpr int PH(int k) { return SL_FIB + 3 * k + 1; }
@@ -177,6 +199,7 @@ struct Chain {
       __sincosf(v<TH(k)>(), &st, &ct);
       __sincosf(v<PH(k)>(), &sp, &cp);
       D.x[k][0] = st * cp; D.x[k][1] = st * sp; D.x[k][2] = ct;
+      xform(D.x[k][0], D.x[k][1], D.x[k][2]);
     });
     return D;
   }
@@ -343,14 +366,14 @@ struct Chain {
   template <int SA, int SI, bool NA, bool NI, bool SH>
   FDT_DEV float mix_new(int t, const Fr& q, const SigC& ca, const SigC& ci, const Dirs& D) const {
     const float4 g = acq[rl + 16 * t];
-    float isn = NI ? iso_new<SI, SH>(t, ci, g.w) : iso[t];
+    float isn = NI ? iso_new<SI, SH>(t, ci, bv(t, g)) : iso[t];
     float m = fmaf(1.0f - q.sum, isn, q.f0);
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       float sk = s[k][t];
       if constexpr (NA) {
         float dp = fmaf(g.x, D.x[k][0], fmaf(g.y, D.x[k][1], g.z * D.x[k][2]));
-        sk = sig<SA>(ca, g.w * dp * dp, g.w);
+        sk = sig<SA>(ca, g.w * dp * dp, bv(t, g));
         cnd(k, t) = sk;
       }
       m = fmaf(q.fk[k], sk, m);
@@ -359,7 +382,7 @@ struct Chain {
   }
   template <int SA, int SI, bool NA, bool NI, bool SH>
   FDT_DEV void store_new(int t, const SigC& ca, const SigC& ci) {
-    float b = acq[rl + 16 * t].w;
+    float b = bv(t, acq[rl + 16 * t]);
     if (NI) iso[t] = iso_new<SI, SH>(t, ci, b);
     if (NA) {
 #pragma unroll
@@ -373,7 +396,7 @@ struct Chain {
   // variant only; no per-slot branch)
   template <bool NI, class F>
   FDT_DEV void by_shells(F&& f) const {
-    if constexpr (NI) {
+    if constexpr (NI && !GN) {
       if (nshell) f(std::true_type{});
       else f(std::false_type{});
     } else f(std::false_type{});
@@ -472,7 +495,8 @@ struct Chain {
     float st, ct, sp, cp;
     __sincosf(tn, &st, &ct);
     __sincosf(pn, &sp, &cp);
-    const float x0 = st * cp, x1 = st * sp, x2 = ct;
+    float x0 = st * cp, x1 = st * sp, x2 = ct;
+    xform(x0, x1, x2);
     const SigC ca = aniso_coef(model, v<SL_D>(), v<SL_DSTD>(), v<SL_R>());
     const float fk = v<FK(k)>(), S0 = v<SL_S0>(), tau = v<SL_TAU>();
     reload_acq();
@@ -483,7 +507,7 @@ struct Chain {
       for (int t = 0; t < VPL; ++t) {
         float4 g = acq[rl + 16 * t];
         float dp = fmaf(g.x, x0, fmaf(g.y, x1, g.z * x2));
-        sn[t] = sig<decltype(SK)::value>(ca, g.w * dp * dp, g.w);
+        sn[t] = sig<decltype(SK)::value>(ca, g.w * dp * dp, bv(t, g));
       }
     });
     Quad qd;
@@ -558,9 +582,11 @@ struct Chain {
 struct ChainLds {
   float4* acq;
   float* cand;
-  int* sh;
+  int* sh;     // null without a shell table (--gradnonlin)
   float* shb;
   float* iso;
+  float* bp;   // --gradnonlin: [16 rows][16 VPL] b values of the rows' voxels
+  float* gd;   // --gradnonlin: [16 rows][9] deviation values
 };
 
 // stage the acquisition of `a` (16*VPL volumes, zero padded) into LDS; the
@@ -569,7 +595,7 @@ template <int VPL>
 FDT_DEV void stage_acq(const McmcArgs& a, const ChainLds& L) {
   for (int i = threadIdx.x; i < 16 * VPL; i += blockDim.x) {
     L.acq[i] = a.acq[i];
-    L.sh[i] = a.nshell ? a.shell[i] : 0;
+    if (L.sh) L.sh[i] = a.nshell ? a.shell[i] : 0;
   }
   if (threadIdx.x < 16) L.shb[threadIdx.x] = a.nshell ? a.shell_b[threadIdx.x] : 0.0f;
 }
@@ -582,19 +608,29 @@ FDT_DEV void stage_acq(const McmcArgs& a, const ChainLds& L) {
 // indexes per-voxel bvals/alpha/beta with posBV, :266, :711).
 template <int K, int VPL, bool RIC, bool GN = false>
 FDT_DEV void run_chain(const McmcArgs& a, int vox, const ChainLds& L) {
-  using C = Chain<K, VPL, RIC>;
+  using C = Chain<K, VPL, RIC, GN>;
   const int lane = __lane_id();
   const int rl = lane & 15;
   const float4* sacq = L.acq;
-  if constexpr (GN) sacq = a.acqv + (size_t)vox * a.acqv_stride;
   C c;
   c.rl = rl;
   c.acq = sacq;
   c.cand = L.cand + (threadIdx.x >> 6) * (K * VPL * 64) + lane;
   c.sh = L.sh;
   c.shb = L.shb;
-  c.isot = L.iso + (threadIdx.x >> 4) * 16;
-  c.nshell = a.nshell;
+  c.isot = GN ? L.iso : L.iso + (threadIdx.x >> 4) * 16;
+  c.nshell = GN ? 0 : a.nshell;
+  if constexpr (GN) {
+    // the row's b values (from the corrected acquisition table) and deviations
+    float* bp = L.bp + (threadIdx.x >> 4) * (16 * VPL);
+    float* gd = L.gd + (threadIdx.x >> 4) * 9;
+    const float4* av = a.acqv + (size_t)vox * a.acqv_stride;
+#pragma unroll
+    for (int t = 0; t < VPL; ++t) bp[rl + 16 * t] = av[rl + 16 * t].w;
+    if (rl < 9) gd[rl] = a.gdev[(size_t)vox * 9 + rl];
+    c.bp = bp;
+    c.gd = gd;
+  }
   c.model = a.model;
   c.incf0 = a.f0 != 0;
   c.ardf0 = a.ardf0 != 0;
@@ -694,13 +730,13 @@ FDT_DEV void run_chain(const McmcArgs& a, int vox, const ChainLds& L) {
 #pragma unroll
         for (int k = 0; k < K; ++k) {
           float dp = fmaf(g.x, D.x[k][0], fmaf(g.y, D.x[k][1], g.z * D.x[k][2]));
-          c.s[k][t] = sig<decltype(SK)::value>(ca, g.w * dp * dp, g.w);
+          c.s[k][t] = sig<decltype(SK)::value>(ca, g.w * dp * dp, c.bv(t, g));
         }
       }
     });
     by_kind(ci.kind, [&](auto SK) {
 #pragma unroll
-      for (int t = 0; t < VPL; ++t) { float b = sacq[rl + 16 * t].w; c.iso[t] = sig<decltype(SK)::value>(ci, b, b); }
+      for (int t = 0; t < VPL; ++t) { float b = c.bv(t, sacq[rl + 16 * t]); c.iso[t] = sig<decltype(SK)::value>(ci, b, b); }
     });
   }
   if constexpr (RIC) {

@@ -86,6 +86,7 @@ struct McmcArgs {
   float* widths_out;    // [vstride][16] final proposal widths, optional
   const float4* acqv;   // --gradnonlin: per-voxel acquisitions [nvox][acqv_stride >= Npad], or null
   int acqv_stride;
+  const float* gdev;    // --gradnonlin: per-voxel deviation values [nvox][9]
 };
 
 // Batched MCMC over a queue of volumes (fdt_queue.hip): work items are runs

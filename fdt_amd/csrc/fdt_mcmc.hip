@@ -9,10 +9,12 @@ template <int K, int VPL, bool RIC, bool GN>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VPL <= 18 ? 2 : 1, 2))) void mcmc_kernel(McmcArgs a) {
   __shared__ float4 sacq[16 * VPL];
   __shared__ float sba[4 * K * VPL * 64];
-  __shared__ int ssh[16 * VPL];
+  __shared__ int ssh[GN ? 1 : 16 * VPL];
   __shared__ float sshb[16];
-  __shared__ float siso[16 * 16];
-  const ChainLds L{sacq, sba, ssh, sshb, siso};
+  __shared__ float siso[GN ? 1 : 16 * 16];
+  __shared__ float sbp[GN ? 16 * 16 * VPL : 1];
+  __shared__ float sgd[GN ? 16 * 9 : 1];
+  const ChainLds L{sacq, sba, GN ? nullptr : ssh, sshb, siso, sbp, sgd};
   stage_acq<VPL>(a, L);
   __syncthreads();
   const int i = blockIdx.x * 16 + (threadIdx.x >> 4);

@@ -256,14 +256,18 @@ def gradnonlin_voxels(name: str, nvox: int, sd: float = 0.03, clean_only: bool =
     acquisitions."""
     v = voxels(name, nvox)
     grad = grad_dev(rng(name, 3000), nvox, sd)
-    bv, bc = corrected_acquisitions(v["bvals"], v["bvecs"], grad)
     model = v["model"]
-    sig = clean_signal(model, bv, bc, v["S0"], v["d"], v["dstd"], v["f"], v["dirs"])
-    g = rng(name, 3001)
-    sigma = v["S0"] / v["snr"]
-    n1 = g.standard_normal(sig.shape) * sigma[:, None]
-    n2 = g.standard_normal(sig.shape) * sigma[:, None]
-    v["clean"] = sig.astype(np.float32)
-    v["data"] = v["clean"] if clean_only else np.sqrt((sig + n1) ** 2 + n2 ** 2).astype(np.float32)
-    v["grad"] = grad
+    clean = np.empty((nvox, v["bvals"].size), np.float32)
+    data = np.empty_like(clean)
+    for c0 in range(0, nvox, CHUNK):
+        sl = slice(c0, min(nvox, c0 + CHUNK))
+        bv, bc = corrected_acquisitions(v["bvals"], v["bvecs"], grad[sl])
+        sig = clean_signal(model, bv, bc, v["S0"][sl], v["d"][sl], v["dstd"][sl], v["f"][sl], v["dirs"][sl])
+        g = rng(name, 3001 + c0 // CHUNK)
+        sigma = (v["S0"][sl] / v["snr"][sl])[:, None]
+        n1 = g.standard_normal(sig.shape) * sigma
+        n2 = g.standard_normal(sig.shape) * sigma
+        clean[sl] = sig
+        data[sl] = clean[sl] if clean_only else np.sqrt((sig + n1) ** 2 + n2 ** 2)
+    v["clean"], v["data"], v["grad"] = clean, data, grad
     return v
