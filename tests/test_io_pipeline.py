@@ -1,0 +1,191 @@
+"""The bedpostx_gpu pipeline's host side on the CPU: the NIfTI-1 reader/writer
+and mask maps of the C ABI (include/fdt_io.h) against an independent numpy
+implementation of the format, the post-processing oracle against what the
+geometry fixes, and the pipeline's argument and file checks (which end before
+any GPU work)."""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import nifti_np as NI
+from oracle import postproc_oracle as PP
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB = os.path.join(ROOT, "fdt_amd", "libfdt_xfibres.so")
+PIPE = os.path.join(ROOT, "fdt_amd", "bin", "bedpostx_gpu")
+FP = C.POINTER(C.c_float)
+
+
+@pytest.fixture(scope="module")
+def io():
+    L = C.CDLL(LIB)
+    L.fdt_image_read.restype = C.c_void_p
+    L.fdt_image_read.argtypes = [C.c_char_p, C.c_char_p, C.c_int]
+    L.fdt_image_free.argtypes = [C.c_void_p]
+    L.fdt_image_dims.argtypes = [C.c_void_p, C.POINTER(C.c_int), FP]
+    L.fdt_image_data.restype = FP
+    L.fdt_image_data.argtypes = [C.c_void_p]
+    L.fdt_image_write.argtypes = [C.c_char_p, C.c_void_p, C.c_int, FP, C.c_float, C.c_float, C.c_char_p, C.c_int]
+    L.fdt_image_mask_count.restype = C.c_long
+    L.fdt_image_mask_count.argtypes = [C.c_void_p]
+    L.fdt_image_mask_matrix.restype = C.c_long
+    L.fdt_image_mask_matrix.argtypes = [C.c_void_p, C.c_void_p, FP]
+    return L
+
+
+def cread(io, path):
+    err = C.create_string_buffer(256)
+    h = io.fdt_image_read(str(path).encode(), err, 256)
+    assert h, err.value.decode()
+    dim = (C.c_int * 8)()
+    pd = (C.c_float * 8)()
+    io.fdt_image_dims(h, dim, pd)
+    shape = tuple(dim[1:1 + dim[0]])
+    n = int(np.prod(shape))
+    a = np.ctypeslib.as_array(io.fdt_image_data(h), shape=(n,)).copy().reshape(shape, order="F")
+    return h, a, list(pd)
+
+
+@pytest.mark.parametrize("dtype,slope,inter,big,ext", [
+    (np.float32, 0.0, 0.0, False, ".nii.gz"), (np.int16, 2.0, 1.0, False, ".nii"), (np.uint8, 0.0, 0.0, True, ".nii.gz"),
+    (np.float64, 1.0, 0.0, True, ".nii"), (np.int32, 0.5, -3.0, False, ".nii.gz")])
+def test_read_matches_numpy_reader(io, tmp_path, dtype, slope, inter, big, ext):
+    g = np.random.default_rng(1)
+    a = (g.uniform(0, 100, (5, 4, 3, 2))).astype(dtype)
+    p = tmp_path / ("img" + ext)
+    NI.write(p, a, slope=slope, inter=inter, big_endian=big)
+    h, got, pd = cread(io, tmp_path / "img")     # FSL name resolution
+    want, hdr = NI.read(p)
+    io.fdt_image_free(h)
+    np.testing.assert_array_equal(got, want)
+    assert pd[1:4] == [2.0, 2.0, 2.0]
+
+
+def test_write_keeps_geometry_and_reads_back(io, tmp_path):
+    g = np.random.default_rng(2)
+    sform = [[-2, 0, 0, 90], [0, 2, 0, -126], [0, 0, 2, -72]]
+    NI.write(tmp_path / "like.nii.gz", np.zeros((6, 5, 4), np.float32), pixdim=(2.0, 2.5, 3.0), sform=sform)
+    like, _, _ = cread(io, tmp_path / "like.nii.gz")
+    vol = g.standard_normal((6, 5, 4, 7)).astype(np.float32)
+    flat = np.ascontiguousarray(vol.transpose(3, 2, 1, 0)).ravel()   # x fastest, volume after volume
+    err = C.create_string_buffer(256)
+    assert io.fdt_image_write(str(tmp_path / "out.nii.gz").encode(), like, 7, flat.ctypes.data_as(FP), -1.0, 3.0,
+                              err, 256) == 0, err.value
+    back, hdr = NI.read(tmp_path / "out.nii.gz")
+    np.testing.assert_array_equal(back, vol)
+    assert hdr["dims"][:5] == (4, 6, 5, 4, 7) and hdr["datatype"] == 16 and hdr["magic"] == b"n+1\0"
+    assert hdr["pixdim"][1:4] == (2.0, 2.5, 3.0) and hdr["cal"] == (-1.0, 3.0)
+    assert hdr["sform_code"] == 1 and np.allclose(np.reshape(hdr["srow"], (3, 4)), sform)
+    # a 3D write
+    assert io.fdt_image_write(str(tmp_path / "o3.nii").encode(), like, 1, flat.ctypes.data_as(FP), 0.0, 0.0, err, 256) == 0
+    b3, h3 = NI.read(tmp_path / "o3.nii")
+    assert h3["dims"][0] == 3
+    np.testing.assert_array_equal(b3, vol[..., 0])
+    io.fdt_image_free(like)
+
+
+def test_mask_matrix_is_fsl_order(io, tmp_path):
+    g = np.random.default_rng(3)
+    data = g.standard_normal((7, 6, 5, 4)).astype(np.float32)
+    mask = (g.uniform(size=(7, 6, 5)) < 0.4).astype(np.uint8)
+    NI.write(tmp_path / "d.nii.gz", data)
+    NI.write(tmp_path / "m.nii.gz", mask)
+    hd, _, _ = cread(io, tmp_path / "d.nii.gz")
+    hm, _, _ = cread(io, tmp_path / "m.nii.gz")
+    n = io.fdt_image_mask_count(hm)
+    assert n == mask.sum()
+    out = np.zeros((n, 4), np.float32)
+    assert io.fdt_image_mask_matrix(hd, hm, out.ctypes.data_as(FP)) == n
+    # FSL's volume4D::matrix(mask): voxels with x fastest, then y, then z
+    xs, ys, zs = np.nonzero(mask.transpose(2, 1, 0))[::-1]
+    np.testing.assert_array_equal(out, data[xs, ys, zs, :])
+    io.fdt_image_free(hd)
+    io.fdt_image_free(hm)
+
+
+def test_read_errors(io, tmp_path):
+    err = C.create_string_buffer(256)
+    assert not io.fdt_image_read(str(tmp_path / "nope").encode(), err, 256)
+    assert b"not found" in err.value
+    (tmp_path / "bad.nii").write_bytes(b"\0" * 400)
+    assert not io.fdt_image_read(str(tmp_path / "bad.nii").encode(), err, 256)
+    NI.write(tmp_path / "short.nii", np.zeros((4, 4, 4), np.float32))
+    raw = (tmp_path / "short.nii").read_bytes()
+    (tmp_path / "short.nii").write_bytes(raw[:-10])
+    assert not io.fdt_image_read(str(tmp_path / "short.nii").encode(), err, 256)
+    assert b"fewer voxels" in err.value
+
+
+# ---------------------------------------------------------------- oracle pins
+def test_dyadic_vectors_pins():
+    g = np.random.default_rng(4)
+    th = g.uniform(0, np.pi, 6)
+    ph = g.uniform(-np.pi, np.pi, 6)
+    # identical samples: the dyad is the direction, no dispersion
+    d, disp = PP.dyadic_vectors(np.repeat(th[:, None], 10, 1), np.repeat(ph[:, None], 10, 1))
+    v = np.stack([np.sin(th) * np.cos(ph), np.sin(th) * np.sin(ph), np.cos(th)], 1)
+    np.testing.assert_allclose(np.abs((d * v).sum(1)), 1.0, atol=1e-6)
+    np.testing.assert_allclose(disp, 0.0, atol=1e-6)
+    # half along x, half along y: largest eigenvalue 1/2 (dispersion 1/2)
+    th2 = np.tile([np.pi / 2, np.pi / 2], (1, 5))
+    ph2 = np.tile([0.0, np.pi / 2], (1, 5))
+    d2, disp2 = PP.dyadic_vectors(th2, ph2)
+    np.testing.assert_allclose(disp2, 0.5, atol=1e-6)
+    assert abs(d2[0, 2]) < 1e-6
+    # 3 along z, 1 along x: principal z, dispersion 1/4
+    d3, disp3 = PP.dyadic_vectors(np.array([[0, 0, 0, np.pi / 2]]), np.zeros((1, 4)))
+    np.testing.assert_allclose(np.abs(d3[0]), [0, 0, 1], atol=1e-6)
+    np.testing.assert_allclose(disp3, 0.25, atol=1e-6)
+
+
+def test_maskdyads_modf_fsum_pins():
+    dy = np.array([[1, 0, 0], [0, 1, 0], [0, 0, 1], [0.6, 0.8, 0]], np.float32)
+    f2 = np.array([0.049999, 0.05, 0.3, 0.0], np.float32)
+    f1 = np.array([0.5, 0.4, 0.0, 0.2], np.float32)
+    t = PP.maskdyads(dy, f2)
+    np.testing.assert_array_equal(t, [[0, 0, 0], [0, 1, 0], [0, 0, 1], [0, 0, 0]])
+    m = PP.modf(t, f2, f1)
+    np.testing.assert_allclose(m, [[0, 0, 0], [0, 0.125, 0], [0, 0, 0], [0, 0, 0]], atol=1e-7)
+    np.testing.assert_array_equal(PP.fsum([f1, f2]), f1 + f2)
+
+
+# ---------------------------------------------------------------- pipeline checks
+def run_pipe(args, cwd):
+    return subprocess.run([PIPE] + args, cwd=cwd, capture_output=True, text=True, timeout=60)
+
+
+def test_pipeline_file_checks(tmp_path):
+    r = run_pipe([], tmp_path)
+    assert r.returncode == 1 and "Usage: bedpostx_gpu" in r.stdout
+    r = run_pipe(["nosuch"], tmp_path)
+    assert r.returncode == 1 and "subject directory nosuch not found" in r.stderr
+    s = tmp_path / "subj"
+    s.mkdir()
+    r = run_pipe([str(s)], tmp_path)
+    assert r.returncode == 1 and "/bvecs not found" in r.stderr
+    (s / "bvecs.txt").write_text("1 0 0\n0 1 0\n0 0 1\n")
+    r = run_pipe([str(s)], tmp_path)
+    assert r.returncode == 1 and "/bvals not found" in r.stderr
+    (s / "bvals").write_text("0 1000 1000\n")
+    r = run_pipe([str(s)], tmp_path)
+    assert r.returncode == 1 and "/data not found" in r.stderr
+    NI.write(s / "data.nii.gz", np.ones((3, 3, 2, 3), np.float32))
+    r = run_pipe([str(s), "-g"], tmp_path)
+    assert r.returncode == 1 and "/grad_dev not found" in r.stderr
+    r = run_pipe([str(s)], tmp_path)
+    assert r.returncode == 1 and "/nodif_brain_mask not found" in r.stderr
+    NI.write(s / "nodif_brain_mask.nii.gz", np.ones((3, 3, 2), np.uint8))
+    r = run_pipe([str(s), "--bogus"], tmp_path)
+    assert r.returncode == 1 and "unknown option: --bogus" in r.stderr
+    (tmp_path / "subj.bedpostX" / "xfms").mkdir(parents=True)
+    (tmp_path / "subj.bedpostX" / "xfms" / "eye.mat").write_text("1\n")
+    r = run_pipe([str(s)], tmp_path)
+    assert r.returncode == 1 and "has already been processed" in r.stderr
+    (tmp_path / "subj.bedpostX" / "xfms" / "eye.mat").unlink()
+    NI.write(s / "data.nii.gz", np.ones((3, 3, 2, 4), np.float32))   # 4 volumes, 3 b values
+    r = run_pipe([str(s)], tmp_path)
+    assert r.returncode == 1 and "must be the same" in r.stderr
+    assert (tmp_path / "subj.bedpostX" / "bvecs").read_text() == "1 0 0\n0 1 0\n0 0 1\n"
