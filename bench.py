@@ -330,6 +330,88 @@ def bench_gradnonlin(args):
     print(json.dumps(line), flush=True)
 
 
+def _write_nii(path, arr, pixdim=(2.0, 2.0, 2.0)):
+    """uncompressed single-file NIfTI-1, float32 or uint8, arr indexed [x, y, z(, t)]"""
+    import struct
+    arr = np.asarray(arr)
+    code = {np.dtype(np.float32): 16, np.dtype(np.uint8): 2}[arr.dtype]
+    h = bytearray(352)
+    struct.pack_into("<i", h, 0, 348)
+    struct.pack_into("<8h", h, 40, *([arr.ndim] + list(arr.shape) + [1] * (7 - arr.ndim)))
+    struct.pack_into("<2h", h, 70, code, arr.dtype.itemsize * 8)
+    struct.pack_into("<8f", h, 76, 1.0, *pixdim, 1.0, 1.0, 1.0, 1.0)
+    struct.pack_into("<f", h, 108, 352.0)
+    h[344:348] = b"n+1\0"
+    with open(path, "wb") as f:
+        f.write(bytes(h))
+        f.write(np.asfortranarray(arr).tobytes(order="F"))
+
+
+def bench_pipeline(args):
+    """The whole bedpostx_gpu pipeline from files: the multi-shell volume as a
+    NIfTI subject (110 x 110 x 72 grid, ellipsoidal mask of 250,968 voxels,
+    N = 288) through fdt_amd/bin/bedpostx_gpu with the bedpostx defaults
+    (nf 3, model 2, bi 1000, nj 1250, se 25) to the .bedpostX images."""
+    import shutil
+    import subprocess
+    import tempfile
+    from fdt_amd import synth
+
+    grid, axes = (110, 110, 72), (48, 48, 26)
+    c = [(n - 1) / 2.0 for n in grid]
+    x, y, z = np.meshgrid(*[np.arange(n) for n in grid], indexing="ij")
+    mask = (((x - c[0]) / axes[0]) ** 2 + ((y - c[1]) / axes[1]) ** 2 + ((z - c[2]) / axes[2]) ** 2) <= 1.0
+    nvox = int(mask.sum())
+    v = synth.voxels("c1", nvox, with_truth=False)
+    N = v["bvals"].size
+    xs, ys, zs = np.nonzero(mask.transpose(2, 1, 0))[::-1]
+    root = tempfile.mkdtemp(prefix="fdt_pipeline_")
+    try:
+        subj = os.path.join(root, "subj")
+        os.makedirs(subj)
+        t0 = time.perf_counter()
+        data = np.zeros(grid + (N,), np.float32)
+        data[xs, ys, zs, :] = v["data"]
+        _write_nii(os.path.join(subj, "data.nii"), data)
+        del data
+        _write_nii(os.path.join(subj, "nodif_brain_mask.nii"), mask.astype(np.uint8))
+        np.savetxt(os.path.join(subj, "bvals"), v["bvals"][None, :])
+        np.savetxt(os.path.join(subj, "bvecs"), v["bvecs"])
+        t_prep = time.perf_counter() - t0
+        pipe = os.path.join(HERE, "fdt_amd", "bin", "bedpostx_gpu")
+        t0 = time.perf_counter()
+        r = subprocess.run([pipe, subj], capture_output=True, text=True, timeout=900)
+        wall = time.perf_counter() - t0
+        if r.returncode != 0:
+            raise RuntimeError(f"bedpostx_gpu failed: {r.stderr[-2000:]}")
+        log = open(os.path.join(root, "subj.bedpostX", "logs", "bedpostx_gpu.log")).read()
+        out_dir = os.path.join(root, "subj.bedpostX")
+        out_bytes = sum(os.path.getsize(os.path.join(out_dir, f)) for f in os.listdir(out_dir)
+                        if f.endswith(".nii.gz"))
+        n_img = sum(1 for f in os.listdir(out_dir) if f.endswith(".nii.gz"))
+    finally:
+        shutil.rmtree(root, ignore_errors=True)
+
+    def field(prefix):
+        for line in log.splitlines():
+            if line.startswith(prefix):
+                return float(line.split(":")[1].split()[0])
+        return None
+    line = {
+        "metric": "voxels/s, 3-fibre multi-shell bedpostx_gpu pipeline, NIfTI files in to .bedpostX images out",
+        "value": nvox / wall, "unit": "voxels/s", "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": wall * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
+        "data": "synthetic (seeded Philox), no published dataset",
+        "config": {"workload": f"bedpostx_gpu <subject> with defaults (nf 3, model 2, bi 1000, nj 1250, se 25): "
+                               f"110x110x72 grid, {nvox} mask voxels, N={N}, uncompressed data.nii in, "
+                               f"{n_img} .nii.gz images out ({out_bytes / 1e6:.1f} MB)",
+                   "s_read_and_mask": field("read + mask"), "s_engine": field("engine"),
+                   "s_write": field("write"), "s_total_in_process": field("total"), "s_wall": wall,
+                   "s_make_subject": t_prep},
+    }
+    print(json.dumps(line), flush=True)
+
+
 def bench_c2(args):
     """Mixed queue of four volumes (N = 33/69/145/288, models 1/2/2/3, nf
     1/2/3/3, 60,000 voxels each): the batched path (concurrent LM, one
@@ -395,11 +477,12 @@ def main():
     ap.add_argument("--nvox", type=int, default=0, help="override the voxel count (default: the full mask)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--workload", default="c1", choices=["c1", "c2", "c3", "c4", "gradnonlin"],
+    ap.add_argument("--workload", default="c1", choices=["c1", "c2", "c3", "c4", "gradnonlin", "pipeline"],
                     help="c1: the multi-shell 3-fibre volume (default); c2: the mixed-acquisition queue; "
                          "c3: train the learned start and compare it with the plain path; c4: the same for "
                          "model 3 + Rician on the 904,550-voxel volume; gradnonlin: the multi-shell volume "
-                         "with per-voxel gradient deviations")
+                         "with per-voxel gradient deviations; pipeline: the bedpostx_gpu pipeline from NIfTI "
+                         "files to the .bedpostX images")
     ap.add_argument("--oracle-voxels", type=int, default=64)
     ap.add_argument("--train-voxels", type=int, default=2097152)
     ap.add_argument("--epochs", type=int, default=10)
@@ -421,6 +504,9 @@ def main():
         return
     if args.workload == "gradnonlin":
         bench_gradnonlin(args)
+        return
+    if args.workload == "pipeline":
+        bench_pipeline(args)
         return
 
     import fdt_amd as F

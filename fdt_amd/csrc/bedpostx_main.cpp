@@ -22,12 +22,10 @@
 #include <unistd.h>
 
 #include <algorithm>
-#include <atomic>
 #include <chrono>
 #include <cstdio>
 #include <cstring>
 #include <fstream>
-#include <future>
 #include <iostream>
 #include <string>
 #include <thread>
@@ -326,23 +324,16 @@ int main(int argc, char* argv[]) {
       for (int i = 0; i < nvox; ++i) fsum[i] += meanf[k][i];
     add("mean_fsumsamples", fsum.data(), 1, 0.0f, 1.0f, false, false);
   }
-  // write in parallel (gzip level 1)
-  unsigned nthreads = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-  std::vector<std::string> errs(outs.size());
-  std::vector<std::thread> pool;
-  std::atomic<size_t> next{0};
-  for (unsigned w = 0; w < nthreads; ++w)
-    pool.emplace_back([&]() {
-      for (size_t j; (j = next++) < outs.size();) {
-        std::string e;
-        if (!fdt::write_nifti(B + "/" + outs[j].name + ".nii.gz", mask, outs[j].nt, outs[j].vol.data(), outs[j].cal_min,
-                              outs[j].cal_max, e))
-          errs[j] = e;
-      }
-    });
-  for (auto& t : pool) t.join();
-  for (auto& e : errs)
-    if (!e.empty()) { std::cerr << e << std::endl; return EXIT_FAILURE; }
+  // FSLOUTPUTTYPE: NIFTI_GZ (FSL's default) or NIFTI; gzip level 1, members
+  // compressed on all threads
+  const char* ot = std::getenv("FSLOUTPUTTYPE");
+  const std::string ext = (ot && std::string(ot) == "NIFTI") ? ".nii" : ".nii.gz";
+  const int nthreads = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  for (auto& r : outs)
+    if (!fdt::write_nifti(B + "/" + r.name + ext, mask, r.nt, r.vol.data(), r.cal_min, r.cal_max, err, nthreads)) {
+      std::cerr << err << std::endl;
+      return EXIT_FAILURE;
+    }
   {
     std::ofstream eye(B + "/xfms/eye.mat");
     eye << "1 0 0 0\n0 1 0 0\n0 0 1 0\n0 0 0 1\n";

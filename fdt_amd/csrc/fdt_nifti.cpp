@@ -4,8 +4,10 @@
 #include <zlib.h>
 #include <sys/stat.h>
 
+#include <atomic>
 #include <cmath>
 #include <cstring>
+#include <thread>
 
 namespace fdt {
 
@@ -53,6 +55,22 @@ struct GzIn {
     return true;
   }
 };
+
+// gzip (level 1) of `n` bytes as one member into `out`
+bool gzip_member(const uint8_t* src, size_t n, std::vector<uint8_t>& out) {
+  z_stream zs;
+  std::memset(&zs, 0, sizeof zs);
+  if (deflateInit2(&zs, 1, Z_DEFLATED, 15 + 16, 8, Z_DEFAULT_STRATEGY) != Z_OK) return false;
+  out.resize(deflateBound(&zs, (uLong)n) + 64);
+  zs.next_in = const_cast<Bytef*>(src);
+  zs.avail_in = (uInt)n;
+  zs.next_out = out.data();
+  zs.avail_out = (uInt)out.size();
+  int rc = deflate(&zs, Z_FINISH);
+  out.resize(zs.total_out);
+  deflateEnd(&zs);
+  return rc == Z_STREAM_END;
+}
 
 template <class T>
 void convert(const uint8_t* src, size_t n, bool sw, float* dst) {
@@ -137,7 +155,7 @@ bool read_nifti(const std::string& path, Nifti& img, std::string& err) {
 }
 
 bool write_nifti(const std::string& path, const Nifti& like, int nt, const float* data, float cal_min, float cal_max,
-                 std::string& err) {
+                 std::string& err, int threads) {
   uint8_t h[H_SIZE + 4];
   std::memset(h, 0, sizeof h);
   put<int32_t>(h, H_SIZEOF, 348);
@@ -164,19 +182,30 @@ bool write_nifti(const std::string& path, const Nifti& like, int nt, const float
   std::memcpy(h + H_MAGIC, "n+1", 4);
   const size_t bytes = like.nxyz() * (size_t)nt * sizeof(float);
   if (ends_with(path, ".gz")) {
-    gzFile f = gzopen(path.c_str(), "wb1");
+    // a multi-member gzip file (members of 4 MB compressed in parallel, as
+    // pigz does): any gzip reader, zlib's gzread included, reads it as one stream
+    const size_t CH = (size_t)4 << 20;
+    const size_t nch = (bytes + CH - 1) / CH;
+    std::vector<std::vector<uint8_t>> mem(nch + 1);
+    bool ok = gzip_member(h, sizeof h, mem[0]);
+    std::atomic<size_t> next{0};
+    std::atomic<bool> good{true};
+    auto work = [&]() {
+      for (size_t j; (j = next++) < nch;) {
+        const uint8_t* p = reinterpret_cast<const uint8_t*>(data) + j * CH;
+        if (!gzip_member(p, std::min(CH, bytes - j * CH), mem[j + 1])) good = false;
+      }
+    };
+    const int nt_workers = (int)std::min<size_t>(nch, (size_t)std::max(1, threads));
+    std::vector<std::thread> pool;
+    for (int w = 1; w < nt_workers; ++w) pool.emplace_back(work);
+    work();
+    for (auto& t : pool) t.join();
+    if (!ok || !good) { err = "compression failed for " + path; return false; }
+    FILE* f = std::fopen(path.c_str(), "wb");
     if (!f) { err = "cannot write " + path; return false; }
-    gzbuffer(f, 1 << 20);
-    bool ok = gzwrite(f, h, sizeof h) == (int)sizeof h;
-    const uint8_t* p = reinterpret_cast<const uint8_t*>(data);
-    size_t left = bytes;
-    while (ok && left) {
-      unsigned chunk = left > (1u << 30) ? (1u << 30) : (unsigned)left;
-      ok = gzwrite(f, p, chunk) == (int)chunk;
-      p += chunk;
-      left -= chunk;
-    }
-    if (gzclose(f) != Z_OK) ok = false;
+    for (auto& m : mem) ok = ok && std::fwrite(m.data(), 1, m.size(), f) == m.size();
+    if (std::fclose(f) != 0) ok = false;
     if (!ok) { err = "error writing " + path; return false; }
     return true;
   }

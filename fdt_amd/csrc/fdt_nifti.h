@@ -38,9 +38,10 @@ std::string image_path(const std::string& base);
 bool read_nifti(const std::string& path, Nifti& img, std::string& err);
 
 // Write `nt` volumes of float32 with the spatial header (dims 1..3, pixdim,
-// units, qform, sform) of `like`; .gz paths are gzip-compressed (level 1).
+// units, qform, sform) of `like`; .gz paths are gzip-compressed (level 1, on
+// `threads` threads).
 bool write_nifti(const std::string& path, const Nifti& like, int nt, const float* data, float cal_min, float cal_max,
-                 std::string& err);
+                 std::string& err, int threads = 1);
 
 // Indices (into one volume) of the voxels where mask != 0, in storage order.
 std::vector<size_t> mask_indices(const Nifti& mask);

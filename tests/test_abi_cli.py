@@ -33,6 +33,14 @@ def test_library_exports_every_declared_symbol(fdt):
     assert sorted(fdt.EXPORTS) == sorted(declared())
 
 
+def test_library_exports_the_image_io(fdt):
+    txt = open(os.path.join(ROOT, "include", "fdt_io.h")).read()
+    names = sorted(set(re.findall(r"^\s*[\w\s\*]+?\b(fdt_\w+)\s*\(", txt, flags=re.M)))
+    assert len(names) == 7, names
+    lib = ctypes.CDLL(fdt.LIB_PATH)
+    assert not [n for n in names if not hasattr(lib, n)]
+
+
 def test_default_options_match_xfibres(fdt):
     o = fdt.default_opts()
     assert (o.nfibres, o.model, o.njumps, o.nburn, o.sampleevery, o.updateproposalevery, o.seed) == \

@@ -79,6 +79,9 @@ def test_write_keeps_geometry_and_reads_back(io, tmp_path):
     assert hdr["dims"][:5] == (4, 6, 5, 4, 7) and hdr["datatype"] == 16 and hdr["magic"] == b"n+1\0"
     assert hdr["pixdim"][1:4] == (2.0, 2.5, 3.0) and hdr["cal"] == (-1.0, 3.0)
     assert hdr["sform_code"] == 1 and np.allclose(np.reshape(hdr["srow"], (3, 4)), sform)
+    h2, again, _ = cread(io, tmp_path / "out.nii.gz")      # the multi-member gzip file through zlib's reader
+    np.testing.assert_array_equal(again, vol)
+    io.fdt_image_free(h2)
     # a 3D write
     assert io.fdt_image_write(str(tmp_path / "o3.nii").encode(), like, 1, flat.ctypes.data_as(FP), 0.0, 0.0, err, 256) == 0
     b3, h3 = NI.read(tmp_path / "o3.nii")
