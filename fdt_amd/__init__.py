@@ -29,7 +29,8 @@ EXPORTS = ["fdt_default_opts", "fdt_create", "fdt_destroy", "fdt_last_error", "f
            "fdt_queue_autotune", "fdt_queue_timing", "fdt_get_widths",
            "fdt_surrogate_create", "fdt_surrogate_destroy", "fdt_surrogate_last_error", "fdt_surrogate_num_weights",
            "fdt_surrogate_load", "fdt_surrogate_save", "fdt_surrogate_set_training_data", "fdt_surrogate_gradients",
-           "fdt_surrogate_train_step", "fdt_surrogate_forward", "fdt_surrogate_infer", "fdt_run_surrogate", "fdt_get_fallback_flags"]
+           "fdt_surrogate_train_step", "fdt_surrogate_forward", "fdt_surrogate_infer", "fdt_run_surrogate", "fdt_get_fallback_flags",
+           "fdt_predict"]
 
 
 class Opts(C.Structure):
@@ -83,6 +84,7 @@ def lib():
         L.fdt_load_data.argtypes = [vp, _FP, C.c_int, C.c_int]
         L.fdt_load_grad_dev.argtypes = [vp, _FP, C.c_int, C.c_int]
         L.fdt_get_voxel_acquisition.argtypes = [vp, C.c_int, _FP, _FP]
+        L.fdt_predict.argtypes = [C.c_int, C.c_int, _FP, _FP, C.c_int, C.c_int, C.c_int] + [_FP] * 8
         L.fdt_num_params.argtypes = [vp]
         L.fdt_fit.argtypes = [vp]
         L.fdt_lm_stage.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float, _FP, _FP, ip]
@@ -472,6 +474,23 @@ class Queue:
         arr, out = _alloc_outputs(o, nvox, raw_scalars)
         self._chk(self.L.fdt_queue_get_outputs(self.h, job, C.byref(out)))
         return arr
+
+
+def predict(bvals, bvecs, S0, d, f, dyads, model=1, dstd=None, R=None, f0=None, device=0):
+    """posterior-predictive signal [N][nvox] from the posterior means (fdt_predict)"""
+    bv, bvp = _f32(bvals)
+    bc, bcp = _f32(np.asarray(bvecs).reshape(3, -1))
+    S0, S0p = _f32(S0)
+    d, dp = _f32(d)
+    f, fp = _f32(np.asarray(f).reshape(-1, S0.size) if S0.size else np.zeros((np.asarray(f).shape[0], 0)))
+    dy, dyp = _f32(np.asarray(dyads).reshape(f.shape[0], 3, S0.size))
+    opt = lambda a: _f32(a)[1] if a is not None else _FP()
+    out = np.zeros((bv.size, S0.size), np.float32)
+    rc = lib().fdt_predict(device, bv.size, bvp, bcp, S0.size, f.shape[0], model, S0p, dp, opt(dstd), opt(R), opt(f0),
+                           fp, dyp, out.ctypes.data_as(_FP))
+    if rc:
+        raise FdtError(f"fdt_predict failed ({rc})")
+    return out
 
 
 def run_xfibres(bvals, bvecs, data, seed=8665904, vox_offset=0, layout=LAYOUT_VOX_MAJOR, grad=None, **opts):

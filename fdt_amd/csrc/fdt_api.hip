@@ -47,6 +47,7 @@ hipError_t fdt_launch_sg_decode(const SgDecodeArgs*, hipStream_t);
 hipError_t fdt_launch_sg_fallback(const SgFallbackArgs*, hipStream_t);
 hipError_t fdt_launch_finish(const FinishArgs*, hipStream_t);
 hipError_t fdt_launch_transpose(const float*, float*, int, int, hipStream_t);
+hipError_t fdt_launch_pred(const PredArgs*, hipStream_t);
 hipError_t fdt_launch_grad_acq(const float4* base, int N, int stride, const float* grad, int nvox, float4* out,
                                hipStream_t s);
 #define FDT_VPL_DECL(V) hipError_t fdt_launch_mcmc_vpl##V(const McmcArgs*, int, int, hipStream_t);
@@ -1687,6 +1688,48 @@ int fdt_get_widths(fdt_ctx* c, float* widths) {
   HIPCHK(c, hipMemcpyAsync(widths, c->d_widths_out.p, sizeof(float) * c->nvox * 16, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return FDT_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ prediction
+// xfibres_pred (xfibres_pred.cc:153-190) on the device: upload the posterior
+// means, one launch, download the [N][nvox] prediction.
+extern "C" {
+
+int fdt_predict(int device, int N, const float* bvals, const float* bvecs, int nvox, int K, int model, const float* S0,
+                const float* d, const float* dstd, const float* R, const float* f0, const float* f, const float* dyads,
+                float* out) {
+  if (N < 1 || nvox < 0 || K < 0 || model < 1 || model > 3) return FDT_EINVAL;
+  if (!bvals || !bvecs || !out || (nvox > 0 && (!S0 || !d || (K > 0 && (!f || !dyads)))) ||
+      (model >= 2 && nvox > 0 && !dstd) || (model == 3 && nvox > 0 && !R))
+    return FDT_EINVAL;
+  if (nvox == 0) return FDT_OK;
+  if (hipSetDevice(device) != hipSuccess) return FDT_EHIP;
+  const size_t nv = (size_t)nvox;
+  DevBuf<float> bv, bc, s0, dd, ds, rr, ff0, ff, dy, o;
+  hipStream_t st = nullptr;
+  auto up = [&](DevBuf<float>& b, const float* h, size_t n) -> bool {
+    if (!h) return true;
+    return b.ensure(n) == hipSuccess && hipMemcpyAsync(b.p, h, n * sizeof(float), hipMemcpyHostToDevice, st) == hipSuccess;
+  };
+  bool ok = hipStreamCreate(&st) == hipSuccess;
+  ok = ok && up(bv, bvals, N) && up(bc, bvecs, 3 * (size_t)N) && up(s0, S0, nv) && up(dd, d, nv);
+  ok = ok && up(ds, model >= 2 ? dstd : nullptr, nv) && up(rr, model == 3 ? R : nullptr, nv) && up(ff0, f0, nv);
+  ok = ok && up(ff, K ? f : nullptr, (size_t)K * nv) && up(dy, K ? dyads : nullptr, (size_t)K * 3 * nv);
+  ok = ok && o.ensure((size_t)N * nv) == hipSuccess;
+  if (ok) {
+    PredArgs a{};
+    a.N = N; a.nvox = nvox; a.K = K; a.model = model;
+    a.bvals = bv.p; a.bvecs = bc.p; a.S0 = s0.p; a.d = dd.p; a.dstd = ds.p; a.R = rr.p; a.f0v = f0 ? ff0.p : nullptr;
+    a.f = ff.p; a.dyads = dy.p; a.out = o.p;
+    ok = fdt_launch_pred(&a, st) == hipSuccess &&
+         hipMemcpyAsync(out, o.p, (size_t)N * nv * sizeof(float), hipMemcpyDeviceToHost, st) == hipSuccess &&
+         hipStreamSynchronize(st) == hipSuccess;
+  }
+  if (st) (void)hipStreamDestroy(st);
+  for (auto* b : {&bv, &bc, &s0, &dd, &ds, &rr, &ff0, &ff, &dy, &o}) b->release();
+  return ok ? FDT_OK : FDT_EHIP;
 }
 
 }  // extern "C"

@@ -153,42 +153,6 @@ __global__ void f0_flag_kernel(F0FlagArgs a) {
   if (bad) a.list[atomicAdd(a.count, 1)] = v;
 }
 
-// --gradnonlin: the acquisition of every voxel through its gradient-deviation
-// tensor L (grad [nvox][9], L(i,j) = grad[3j+i]), correct_bvals_bvecs of
-// CUDA/xfibres_gpu.cu:151-170: for b > 0, v -> (I+L) v / |(I+L) v| and
-// b -> |(I+L) v|^2 b.  Products and left-to-right sums in double with explicit
-// rounding (no fused multiply-add), |.| rounded to float as the reference's
-// `float mag`, results rounded to float as the reference's float host vectors.
-// One thread per (voxel, volume slot); slots from N to stride are zero.
-__global__ void grad_acq_kernel(const float4* __restrict__ base, int N, int stride, const float* __restrict__ grad,
-                                int nvox, float4* __restrict__ out) {
-  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (size_t)nvox * stride) return;
-  const int vox = (int)(t / stride), i = (int)(t % stride);
-  float4 q = i < N ? base[i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-  if (i < N && q.w > 0.0f) {
-    const float* g = grad + (size_t)vox * 9;
-    const double v[3] = {q.x, q.y, q.z};
-    double w[3];
-#pragma unroll
-    for (int r = 0; r < 3; ++r) {
-      double m0 = (r == 0 ? __dadd_rn(1.0, (double)g[0]) : (double)g[r]);
-      double m1 = (r == 1 ? __dadd_rn(1.0, (double)g[4]) : (double)g[3 + r]);
-      double m2 = (r == 2 ? __dadd_rn(1.0, (double)g[8]) : (double)g[6 + r]);
-      w[r] = __dadd_rn(__dadd_rn(__dmul_rn(m0, v[0]), __dmul_rn(m1, v[1])), __dmul_rn(m2, v[2]));
-    }
-    double ss = __dadd_rn(__dadd_rn(__dmul_rn(w[0], w[0]), __dmul_rn(w[1], w[1])), __dmul_rn(w[2], w[2]));
-    float mag = __double2float_rn(sqrt(ss));
-    if (mag != 0.0f)
-#pragma unroll
-      for (int r = 0; r < 3; ++r) w[r] = __ddiv_rn(w[r], (double)mag);
-    float m2 = __fmul_rn(mag, mag);
-    q = make_float4(__double2float_rn(w[0]), __double2float_rn(w[1]), __double2float_rn(w[2]),
-                    __double2float_rn(__dmul_rn((double)m2, (double)q.w)));
-  }
-  out[t] = q;
-}
-
 // work-queue counters to zero with a kernel (capturable in a graph)
 __global__ void zero_ints_kernel(int* p, int n) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -223,15 +187,5 @@ extern "C" hipError_t fdt_launch_transpose(const float* in, float* out, int N, i
   if (nvox <= 0 || N <= 0) return hipSuccess;
   dim3 grid((nvox + 31) / 32, (N + 31) / 32), block(256);
   hipLaunchKernelGGL(fdt::transpose_kernel, grid, block, 0, s, in, out, N, nvox);
-  return hipGetLastError();
-}
-
-extern "C" hipError_t fdt_launch_grad_acq(const float4* base, int N, int stride, const float* grad, int nvox,
-                                          float4* out, hipStream_t s) {
-  if (nvox <= 0) return hipSuccess;
-  if (N > stride) return hipErrorInvalidValue;
-  size_t n = (size_t)nvox * stride;
-  hipLaunchKernelGGL(fdt::grad_acq_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, base, N, stride, grad,
-                     nvox, out);
   return hipGetLastError();
 }

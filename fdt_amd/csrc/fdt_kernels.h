@@ -104,6 +104,17 @@ struct QueueArgs {
   int* counter;           // work-queue counter, zero at launch
 };
 
+// posterior-predictive signal from the posterior means (xfibres_pred.cc:153-190)
+struct PredArgs {
+  int N, nvox, K, model;
+  const float* bvals;   // [N]
+  const float* bvecs;   // [3][N]
+  const float *S0, *d, *dstd, *R, *f0v;  // [nvox]; dstd (model >= 2), R (model 3), f0v (or null)
+  const float* f;       // [K][nvox] mean f
+  const float* dyads;   // [K][3][nvox]
+  float* out;           // [N][nvox]
+};
+
 struct FinishArgs {
   int K, ns, nvox, model, f0, rician;
   const float *th, *ph, *f, *S0, *d, *dstd, *R, *f0s, *taus;  // raw samples

@@ -262,6 +262,18 @@ int fdt_run_surrogate(fdt_ctx* c, fdt_surrogate* s, uint64_t vox_offset, int nbu
  * squares, bit 2 direction (0: kept) */
 int fdt_get_fallback_flags(fdt_ctx* c, unsigned char* flags);
 
+/* ------------------------------------------------------ posterior prediction
+ * The signal predicted by the posterior means (xfibres_pred.cc:153-190, the
+ * per-voxel loop of xfibres_pred): for every volume l and voxel i,
+ * S0 (sum_k f_k stick_k + (1 - sum f - f0) ball + f0) with the model's stick
+ * and ball forms (model 1; model 2 gamma diffusivities; model 3 zeppelins),
+ * evaluated on device `device`.  Inputs per voxel [nvox] (dstd for model >= 2,
+ * R for model 3, f0 optional), f [K][nvox], dyads [K][3][nvox]; bvals [N],
+ * bvecs 3 x N unit vectors; out [N][nvox]. */
+int fdt_predict(int device, int N, const float* bvals, const float* bvecs, int nvox, int K, int model, const float* S0,
+                const float* d, const float* dstd, const float* R, const float* f0, const float* f, const float* dyads,
+                float* out);
+
 #ifdef __cplusplus
 }
 #endif
