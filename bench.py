@@ -571,7 +571,7 @@ def main():
                    "mcmc_tile": f"{tm['lanes']} lanes x {tm['vpl']} volumes per voxel"},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP32_PEAK_TFLOPS, "traffic": traffic,
-                     "kernel": "mcmc_kernel<3,18,false>", "flops_per_launch": fl_launch,
+                     "kernel": "mcmc_kernel<3,18,false,false>", "flops_per_launch": fl_launch,
                      "compute": "FP32 vector ALU (the path has no matrix-shaped work); peak is the published FP32 "
                                 "figure, which counts packed FP32 -- this build disables packed FP32, so half of it "
                                 "is the reachable issue ceiling"},
