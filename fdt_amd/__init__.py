@@ -26,7 +26,7 @@ EXPORTS = ["fdt_default_opts", "fdt_create", "fdt_destroy", "fdt_last_error", "f
            "fdt_get_timing", "fdt_set_mcmc_tile",
            "fdt_queue_create", "fdt_queue_destroy", "fdt_queue_last_error", "fdt_enqueue_batch", "fdt_run_queue",
            "fdt_queue_num_jobs", "fdt_queue_get_outputs", "fdt_queue_get_params", "fdt_queue_set_tiles",
-           "fdt_queue_autotune", "fdt_queue_timing", "fdt_get_widths",
+           "fdt_queue_autotune", "fdt_queue_timing", "fdt_get_widths", "fdt_queue_set_grad_dev",
            "fdt_surrogate_create", "fdt_surrogate_destroy", "fdt_surrogate_last_error", "fdt_surrogate_num_weights",
            "fdt_surrogate_load", "fdt_surrogate_save", "fdt_surrogate_set_training_data", "fdt_surrogate_gradients",
            "fdt_surrogate_train_step", "fdt_surrogate_forward", "fdt_surrogate_infer", "fdt_run_surrogate", "fdt_get_fallback_flags",
@@ -110,6 +110,7 @@ def lib():
         L.fdt_queue_get_outputs.argtypes = [vp, C.c_int, C.POINTER(Outputs)]
         L.fdt_queue_get_params.argtypes = [vp, C.c_int, _FP, _FP]
         L.fdt_queue_set_tiles.argtypes = [vp, C.c_int, C.c_int, C.c_int]
+        L.fdt_queue_set_grad_dev.argtypes = [vp, C.c_int, _FP, C.c_int]
         L.fdt_queue_autotune.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_float)]
         L.fdt_queue_timing.argtypes = [vp, C.POINTER(C.c_float), ip, ip, C.c_int]
         L.fdt_get_widths.argtypes = [vp, _FP]
@@ -444,6 +445,14 @@ class Queue:
 
     def run(self):
         self._chk(self.L.fdt_run_queue(self.h))
+
+    def set_grad_dev(self, job, grad, layout=LAYOUT_VOX_MAJOR):
+        """--gradnonlin for one job: deviations [nvox][9] (None clears)"""
+        if grad is None:
+            self._chk(self.L.fdt_queue_set_grad_dev(self.h, job, _FP(), layout))
+        else:
+            g, gp = _f32(grad)
+            self._chk(self.L.fdt_queue_set_grad_dev(self.h, job, gp, layout))
 
     def set_tiles(self, rows_per_wg=0, job=-1, vpl=0):
         self._chk(self.L.fdt_queue_set_tiles(self.h, rows_per_wg, job, vpl))

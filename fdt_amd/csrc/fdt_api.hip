@@ -943,9 +943,11 @@ static void queue_invalidate(fdt_queue* q) {
   q->graph = nullptr;
 }
 
+// jobs the queue kernel carries; Rician, --gradnonlin and the widest tiles run
+// their own chain launch inside the batch
 static bool in_queue_kernel(const fdt_ctx* c) {
   int vpl = tile_of(c->N, c->vpl_force);
-  return !c->o.rician && vpl > 0 && vpl <= QUEUE_MAX_VPL;
+  return !c->o.rician && !c->gn && vpl > 0 && vpl <= QUEUE_MAX_VPL;
 }
 
 static int queue_launch_chains(fdt_queue* q, const QueueJob* d_jobs, int njobs, const int* d_first, int total) {
@@ -1063,6 +1065,17 @@ int fdt_enqueue_batch(fdt_queue* q, const fdt_opts* o, int N, const float* bvals
   queue_invalidate(q);
   q->ran = false;
   return (int)q->jobs.size() - 1;
+}
+
+int fdt_queue_set_grad_dev(fdt_queue* q, int job, const float* grad, int layout) {
+  if (!q) return FDT_EINVAL;
+  if (job < 0 || job >= (int)q->jobs.size()) return qerr(q, FDT_EINVAL, "job %d out of range", job);
+  (void)hipSetDevice(q->device);
+  fdt_ctx* c = q->jobs[job];
+  if (int rc = fdt_load_grad_dev(c, grad, layout, c->nvox)) return qerr(q, rc, "job %d: %s", job, c->err.c_str());
+  queue_invalidate(q);
+  q->ran = false;
+  return FDT_OK;
 }
 
 int fdt_queue_set_tiles(fdt_queue* q, int rows_per_wg, int job, int vpl) {

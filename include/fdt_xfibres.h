@@ -211,6 +211,12 @@ int fdt_run_queue(fdt_queue* q);
 int fdt_queue_num_jobs(const fdt_queue* q);
 int fdt_queue_get_outputs(fdt_queue* q, int job, const fdt_outputs* out);
 int fdt_queue_get_params(fdt_queue* q, int job, float* params, float* tau);
+/* --gradnonlin for job `job` of the queue: its gradient deviations
+ * ([9][nvox] with FDT_LAYOUT_VOL_MAJOR, as fdt_load_grad_dev; NULL clears).
+ * Such a job runs its chains in its own launch inside the batch.  No
+ * counterpart in the reference (its parts run one process each). */
+int fdt_queue_set_grad_dev(fdt_queue* q, int job, const float* grad, int layout);
+
 /* Voxels per workgroup of the queue kernel (4, 8, 16; 0 = keep) and the
  * volumes-per-lane tile of one job (job < 0: none; vpl 0 = automatic). */
 int fdt_queue_set_tiles(fdt_queue* q, int rows_per_wg, int job, int vpl);

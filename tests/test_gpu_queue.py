@@ -117,3 +117,27 @@ def test_queue_errors(fdt):
             q.set_tiles(rows_per_wg=5)
     finally:
         q.close()
+
+
+def test_gradnonlin_job_in_a_batch(fdt):
+    # a --gradnonlin volume among plain ones: its chains run in their own launch,
+    # every job still equals its per-volume run, and the graph replay repeats it
+    q, vols = build_queue(fdt, 36)
+    try:
+        v = synth.gradnonlin_voxels("c0", 36, sd=0.04)
+        j = q.enqueue(v["bvals"], v["bvecs"], v["data"], vox_offset=5000, nfibres=2, model=1, nburn=20, njumps=50,
+                      sampleevery=25)
+        q.set_grad_dev(j, v["grad"])
+        q.run()
+        first = q.outputs(j)
+        want = fdt.run_xfibres(v["bvals"], v["bvecs"], v["data"], vox_offset=5000, grad=v["grad"], nfibres=2, model=1,
+                               nburn=20, njumps=50, sampleevery=25)
+        for k in ("th", "ph", "f", "mean_S0", "dyads"):
+            np.testing.assert_array_equal(first[k], want[k], err_msg=k)
+        for jj, (vv, kw) in enumerate(vols):
+            ref = fdt.run_xfibres(vv["bvals"], vv["bvecs"], vv["data"], vox_offset=1000 * jj, **kw)
+            np.testing.assert_array_equal(q.outputs(jj)["th"], ref["th"])
+        q.run()  # replay
+        np.testing.assert_array_equal(q.outputs(j)["th"], first["th"])
+    finally:
+        q.close()
