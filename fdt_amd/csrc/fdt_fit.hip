@@ -52,6 +52,9 @@ struct Mdl {
   T fs[K], x0[K], x1[K], x2[K], sth[K], cth[K], sph[K], cph[K];
   T fder[K][K];
   T sumf, d, lam, a, bb, tf0, part;
+  // per-stage constants of the Jacobian rows, hoisted out of the volume loop:
+  // d f / d x of the f = x2f(x) fractions (and of f0), and 1 / p[0]
+  T dfx[K], dfx0, ip0;
   int G;
   T Rv, invR;
 
@@ -112,6 +115,15 @@ struct Mdl {
       for (int k = 0; k < K; ++k) { fs[k] = x2f<T>(p[3 + 3 * k]); sumf += fs[k]; }
       if constexpr (F0) tf0 = x2f<T>(p[NP - 1]);
     }
+    if constexpr (KIND != KIND_SINGLE_C) {
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        const T x = p[FB + 3 * k];
+        dfx[k] = T(TWO_PI_D) * sgn<T>(x) * T(1) / (T(1) + x * x);
+      }
+      if constexpr (F0) dfx0 = T(TWO_PI_D) * sgn<T>(p[NP - 1]) * T(1) / (T(1) + p[NP - 1] * p[NP - 1]);
+    }
+    ip0 = T(1) / p[0];
   }
 
   FDT_DEV T aniso(T b, T dp) const {
@@ -160,19 +172,19 @@ struct Mdl {
         T dp1p = sth[k] * (-gx * sph[k] + gy * cph[k]);
         sig += fs[k] * e;
         Jg[1] += sg1 * S0 * fs[k] * (-b * dp * dp * e);
-        Jg[kk] = S0 * (e - iso) * T(TWO_PI_D) * sgn<T>(p[kk]) * T(1) / (T(1) + p[kk] * p[kk]);
+        Jg[kk] = S0 * (e - iso) * dfx[k];
         Jg[kk + 1] = S0 * fs[k] * (T(-2) * b * d * dp * dp1t * e);
         Jg[kk + 2] = S0 * fs[k] * (T(-2) * b * d * dp * dp1p * e);
       }
       if constexpr (F0) {
-        Jg[NP - 1] = S0 * (T(1) - iso) * T(TWO_PI_D) * sgn<T>(p[NP - 1]) * T(1) / (T(1) + p[NP - 1] * p[NP - 1]);
+        Jg[NP - 1] = S0 * (T(1) - iso) * dfx0;
         sig = S0 * ((tf0 + (T(1) - sumf - tf0) * iso) + sig);
         Jg[1] += sg1 * S0 * (T(1) - sumf - tf0) * iso_d;
       } else {
         sig = S0 * ((T(1) - sumf) * iso + sig);
         Jg[1] += sg1 * S0 * (T(1) - sumf) * iso_d;
       }
-      Jg[0] = sig / S0;
+      Jg[0] = sig * ip0;
       J0h = Jg[0];
     } else if constexpr (KIND == KIND_SINGLE_C) {
       T S0 = p[0];
@@ -188,7 +200,7 @@ struct Mdl {
         int kk = 2 + 3 * k;
         T dp = dps[k];
         sig += fs[k] * A[k];
-        Jg[1] += S0 * fs[k] * (T(-2) * b * lam * dp * dp * exp(-b * lam * lam * dp * dp));
+        Jg[1] += S0 * fs[k] * (T(-2) * b * lam * dp * dp * A[k]);  // exp(-b lam^2 (g.x)^2) = A[k]
         T jb = T(0);
 #pragma unroll
         for (int j = 0; j < K; ++j)
@@ -208,7 +220,7 @@ struct Mdl {
         sig = S0 * ((T(1) - sumf) * iso + sig);
         Jg[1] += S0 * (T(1) - sumf) * isol;
       }
-      Jg[0] = sig / S0;
+      Jg[0] = sig * ip0;
       J0h = Jg[0];
     } else {
       T S0a = fabs(p[0]);
@@ -243,23 +255,24 @@ struct Mdl {
           an_ph = -(T(2) * b * T(3) * a * bb * invR * (T(1) - Rv) * dp1p) * an;
         } else {
           T den = T(1) + b * (dp * dp) * bb;
+          T rden = T(1) / den;
           T l1 = log(den);
           an = exp(-a * l1);
           an_a = -l1 * an;
-          an_b = -a * b * (dp * dp) / den * an;
-          T c = -a * bb * b / den * an;
+          an_b = -a * b * (dp * dp) * rden * an;
+          T c = -a * bb * b * rden * an;
           an_th = c * T(2) * dp * dp1t;
           an_ph = c * T(2) * dp * dp1p;
         }
         sig += fs[k] * an;
         Jg[1] += sg1 * S0a * fs[k] * an_a;
         Jg[2] += sg2 * S0a * fs[k] * an_b;
-        Jg[kk] = S0a * (an - iso) * T(TWO_PI_D) * sgn<T>(p[kk]) * T(1) / (T(1) + p[kk] * p[kk]);
+        Jg[kk] = S0a * (an - iso) * dfx[k];
         Jg[kk + 1] = S0a * fs[k] * an_th;
         Jg[kk + 2] = S0a * fs[k] * an_ph;
       }
       if constexpr (F0) {
-        Jg[NP - 1] = S0a * (T(1) - iso) * T(TWO_PI_D) * sgn<T>(p[NP - 1]) * T(1) / (T(1) + p[NP - 1] * p[NP - 1]);
+        Jg[NP - 1] = S0a * (T(1) - iso) * dfx0;
         sig = S0a * ((tf0 + (T(1) - sumf - tf0) * iso) + sig);
         Jg[1] += sg1 * S0a * (T(1) - sumf - tf0) * iso_a;
         Jg[2] += sg2 * S0a * (T(1) - sumf - tf0) * iso_b;
@@ -269,8 +282,8 @@ struct Mdl {
         Jg[2] += sg2 * S0a * (T(1) - sumf) * iso_b;
       }
       T sg0 = p[0] > T(0) ? T(1) : T(-1);
-      Jg[0] = sg0 * sig / p[0];
-      J0h = sig / p[0];
+      J0h = sig * ip0;
+      Jg[0] = sg0 * J0h;
     }
     return sig;
   }
@@ -369,8 +382,10 @@ FDT_DEV void row_normal_eq(const Row& R, const Mdl<T, KIND, K, F0>& m, const T* 
     T Jg[NP], J0h;
     T sig = m.jrow(p, (T)q.w, (T)q.x, (T)q.y, (T)q.z, Jg, J0h);
     T diff = sig - (T)R.y[i];
+    // J^T r and J^T J; the factor 2 of the gradient and Hessian is applied
+    // after the row sums
 #pragma unroll
-    for (int a = 0; a < NP; ++a) gacc[a] += T(2) * Jg[a] * diff;
+    for (int a = 0; a < NP; ++a) gacc[a] += Jg[a] * diff;
     int h = 0;
 #pragma unroll
     for (int a = 0; a < NP; ++a) {
@@ -378,14 +393,14 @@ FDT_DEV void row_normal_eq(const Row& R, const Mdl<T, KIND, K, F0>& m, const T* 
 #pragma unroll
       for (int c = a; c < NP; ++c) {
         T jc = c == 0 ? J0h : Jg[c];
-        hacc[h++] += T(2) * (ja * jc);
+        hacc[h++] += ja * jc;
       }
     }
   }
   T hsel[NP], gsel = T(0);
 #pragma unroll
   for (int a = 0; a < NP; ++a) {
-    T ga = rsum<T>(gacc[a]);
+    T ga = T(2) * rsum<T>(gacc[a]);
     gsel = R.rl == a ? ga : gsel;
   }
   int h = 0;
@@ -393,7 +408,7 @@ FDT_DEV void row_normal_eq(const Row& R, const Mdl<T, KIND, K, F0>& m, const T* 
   for (int a = 0; a < NP; ++a)
 #pragma unroll
     for (int c = a; c < NP; ++c) {
-      T hv = rsum<T>(hacc[h++]);
+      T hv = T(2) * rsum<T>(hacc[h++]);
       if (R.rl == a) hsel[c] = hv;
       if (R.rl == c) hsel[a] = hv;
     }
