@@ -1567,7 +1567,8 @@ int fdt_surrogate_infer(fdt_surrogate* s, fdt_ctx* c) {
 // Learned start + short burn-in, with the plain path as fallback: the network
 // gives start values and proposal widths, the chains run nburn_short burn-in
 // sweeps and the recorded ones; voxels whose final sum of squares exceeds
-// chi2_max * N * sigma^2 (sigma^2: variance of the b=0 volumes) or whose
+// chi2_max * N * sigma^2 (sum of squares of the final state's residuals y - S0 M
+// for either noise model; sigma^2: variance of the b=0 volumes) or whose
 // posterior mean dyad of fibre 1 lies more than ang_max_deg from the predicted
 // direction are fitted with the plain LM and rerun with the configured
 // burn-in.  *n_fallback receives the number of rerun voxels.
@@ -1591,6 +1592,8 @@ int fdt_run_surrogate(fdt_ctx* c, fdt_surrogate* s, uint64_t vox_offset, int nbu
   mcmc_launch_fn fn = nullptr;
   if (int rc = mcmc_prepare(c, vox_offset, &a, &fn)) return rc;
   a.nburn = nburn_short;
+  a.energy = nullptr;
+  a.ssr = c->d_energy.p;  // the sum-of-squares test below, for either noise model
   HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
   HIPCHK(c, fn(&a, K, o.rician, c->stream));
   c->have_samples = true;
@@ -1611,8 +1614,9 @@ int fdt_run_surrogate(fdt_ctx* c, fdt_surrogate* s, uint64_t vox_offset, int nbu
   HIPCHK(c, c->d_refit.ensure(std::max(nv, 1)));
   fa.flag = c->d_refit.p;
   fa.dir = dir.p;
-  // the sum-of-squares test needs Gaussian energies; Rician chains use the direction test only
-  fa.chi2_max = o.rician ? 3.0e38f : chi2_max;
+  // sum of squares of the short chain's final residuals against chi2_max N sigma^2 (for Rician data
+  // the residual of the mean signal, which the noise floor inflates little at the SNR of diffusion MRI)
+  fa.chi2_max = chi2_max;
   fa.cos_min = std::cos(ang_max_deg * 3.14159265358979f / 180.0f);
   fa.list = c->d_list.p;
   fa.count = c->d_queue.p + 7;

@@ -795,6 +795,22 @@ FDT_DEV void run_chain(const McmcArgs& a, int vox, const ChainLds& L) {
     float tau = c.template v<SL_TAU>();
     if (rl == 0) a.energy[vox] = RIC ? (float)(-(double)a.N * (double)flog(tau) - c.like) : (float)c.like;
   }
+  if (a.ssr) {  // sum of squares of the final state's residuals (the Rician chain keeps the data in r)
+    double ss;
+    if constexpr (RIC) {
+      const float S0 = c.template v<SL_S0>();
+      float acc = 0.0f;
+#pragma unroll
+      for (int t = 0; t < VPL; ++t) {
+        const float e = c.r[t] - S0 * c.M[t];
+        acc += c.mask_last(t, e * e);
+      }
+      ss = (double)row_sum16(acc);
+    } else {
+      ss = c.like;
+    }
+    if (rl == 0) a.ssr[vox] = (float)ss;
+  }
   if (a.widths_out) a.widths_out[(size_t)vox * 16 + rl] = c.sprop;
 }
 

@@ -80,6 +80,7 @@ struct McmcArgs {
   // raw samples: th/ph/f [K][ns][nvox]; scalars [ns][nvox]
   float *th, *ph, *f, *S0, *d, *dstd, *R, *f0s, *taus;
   float* energy;        // [nvox] final SSR (Gaussian) / likelihood energy, optional
+  float* ssr;           // [nvox] final sum of squared residuals y - S0 M (either noise model), optional
   const int* list;      // chains to run: voxels list[0..nvox), or 0..nvox-1 when null
   int vstride;          // voxel count of the sample arrays (>= every voxel index)
   const float* widths_in;  // [vstride][16] initial proposal width per slot (<= 0: default), optional
