@@ -1501,7 +1501,6 @@ static int sg_infer(fdt_surrogate* s, fdt_ctx* c, DevBuf<float>& dir) {
   const fdt_opts& o = c->o;
   const int np = np_of(o), nv = c->nvox;
   if (s->n_in != c->N) return serr(s, FDT_EINVAL, "network input %d, acquisition %d", s->n_in, c->N);
-  if (o.f0) return serr(s, FDT_EUNSUP, "the learned start does not model --f0");
   SHIP(s, c->d_params.ensure((size_t)std::max(nv, 1) * np));
   SHIP(s, c->d_widths_in.ensure((size_t)std::max(nv, 1) * 16));
   SHIP(s, dir.ensure((size_t)std::max(nv, 1) * 3));
@@ -1541,7 +1540,7 @@ static int sg_infer(fdt_surrogate* s, fdt_ctx* c, DevBuf<float>& dir) {
     t.params = c->d_params.p;
     t.np = np;
     t.model = o.model;
-    t.f0 = 0;
+    t.f0 = o.f0;
     t.R = o.R_prior_mean;
     t.tau = c->d_tau.p;
     SHIP(s, launch_tau(t, o.nfibres, 0, c->stream));
@@ -1627,7 +1626,8 @@ int fdt_run_surrogate(fdt_ctx* c, fdt_surrogate* s, uint64_t vox_offset, int nbu
   if (n_fallback) *n_fallback = nfb;
   if (nfb > 0) {
     // plain LM on the listed voxels (stage 1 from the tensor, then PVM_multi for model >= 2)
-    const int np = np_of(o), np1 = 2 + 3 * K;
+    const int f0 = o.f0 ? 1 : 0;
+    const int np = np_of(o), np1 = 2 + 3 * K + f0;
     int* q = c->d_queue.p;
     HIPCHK(c, fdt_launch_zero_ints(q, 4, c->stream));
     LmArgs l = base_lm_args(c);
@@ -1640,12 +1640,12 @@ int fdt_run_surrogate(fdt_ctx* c, fdt_surrogate* s, uint64_t vox_offset, int nbu
     if (o.model == 1) {
       l.out = c->d_params.p;
       l.out_np = np;
-      HIPCHK(c, launch_lm(l, K, o.lm_fp64 ? 1 : 0, k1, 0, c->stream));
+      HIPCHK(c, launch_lm(l, K, o.lm_fp64 ? 1 : 0, k1, f0, c->stream));
     } else {
       HIPCHK(c, c->d_s1.ensure((size_t)nv * np1));
       l.out = c->d_s1.p;
       l.out_np = np1;
-      HIPCHK(c, launch_lm(l, K, o.lm_fp64 ? 1 : 0, k1, 0, c->stream));
+      HIPCHK(c, launch_lm(l, K, o.lm_fp64 ? 1 : 0, k1, f0, c->stream));
       LmArgs m = l;
       m.init = INIT_STAGE1;
       m.in = c->d_s1.p;
@@ -1654,7 +1654,7 @@ int fdt_run_surrogate(fdt_ctx* c, fdt_surrogate* s, uint64_t vox_offset, int nbu
       m.queue = q + 1;
       m.out = c->d_params.p;
       m.out_np = np;
-      HIPCHK(c, launch_lm(m, K, o.lm_fp64 ? 1 : 0, KIND_MULTI, 0, c->stream));
+      HIPCHK(c, launch_lm(m, K, o.lm_fp64 ? 1 : 0, KIND_MULTI, f0, c->stream));
     }
     if (o.rician) {
       // tau from the LM fits (recomputed for every voxel; only the rerun ones read it again)
@@ -1666,6 +1666,7 @@ int fdt_run_surrogate(fdt_ctx* c, fdt_surrogate* s, uint64_t vox_offset, int nbu
       t.params = c->d_params.p;
       t.np = np;
       t.model = o.model;
+      t.f0 = f0;
       t.R = o.R_prior_mean;
       t.tau = c->d_tau.p;
       HIPCHK(c, launch_tau(t, K, o.lm_fp64 ? 1 : 0, c->stream));

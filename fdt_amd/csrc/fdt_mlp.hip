@@ -237,7 +237,8 @@ __global__ void sg_decode_kernel(SgDecodeArgs a) {
     float* w = a.widths + (size_t)v * 16;
     for (int s = 0; s < 16; ++s) w[s] = -1.0f;  // default width
     const int slot_of[14] = {0, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14};
-    for (int i = 0; i < 14; ++i) {
+    // slot 0 (f0) keeps the default width: the training targets carry no f0
+    for (int i = 1; i < 14; ++i) {
       float lw = fminf(fmaxf(o[14 + i], -20.0f), 5.0f);
       w[slot_of[i]] = expf(lw);
     }

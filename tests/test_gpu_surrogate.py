@@ -147,3 +147,38 @@ def test_learned_start_gains_over_plain_at_short_burnin(fdt):
         assert ml <= mf + 0.5
     finally:
         s.close()
+
+
+def test_learned_start_with_f0(fdt):
+    """--f0 from the committed multi-shell weights (trained without an f0
+    target: the chain starts f0 at 0.001 with the default width): the learned
+    path runs every option of the plain one, its fallback fits f0 in the LM,
+    and its dyads stay within 0.3 deg (median) of the plain path with f0."""
+    import os
+    from fdt_amd import surrogate as SG
+    ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    wz = np.load(os.path.join(ROOT, "fdt_amd", "weights", "surrogate_c1_model2_nf3.npz"))
+    ev = synth.voxels("c1", 4096, sub=3)
+    run = dict(nfibres=3, model=2, njumps=1250, sampleevery=25, f0=1, ardf0=1)
+    s = fdt.Surrogate(ev["bvals"].size, seed=0)
+    try:
+        s.load(wz["weights"])
+        plain = fdt.run_xfibres(ev["bvals"], ev["bvecs"], ev["data"], nburn=1000, **run)
+        x = fdt.Xfibres(nburn=1000, **run)
+        try:
+            x.set_acquisition(ev["bvals"], ev["bvecs"])
+            x.load_data(ev["data"])
+            nfb = x.run_surrogate(s, nburn_short=200)
+            learned = x.outputs()
+        finally:
+            x.close()
+        assert np.all(np.isfinite(learned["mean_f0"])) and np.all((learned["mean_f0"] >= 0) & (learned["mean_f0"] < 1))
+        e_p = SG.dyad_errors(plain["dyads"], plain["mean_f"], ev["dirs"], ev["f"])
+        e_l = SG.dyad_errors(learned["dyads"], learned["mean_f"], ev["dirs"], ev["f"])
+        print(f"--f0: median dyad error plain {np.nanmedian(e_p):.3f}, learned {np.nanmedian(e_l):.3f} deg; "
+              f"mean f0 plain {plain['mean_f0'].mean():.4f}, learned {learned['mean_f0'].mean():.4f}; "
+              f"fallback {nfb}/4096")
+        assert np.nanmedian(e_l) <= np.nanmedian(e_p) + 0.3
+        assert abs(learned["mean_f0"].mean() - plain["mean_f0"].mean()) < 0.02
+    finally:
+        s.close()
