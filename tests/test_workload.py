@@ -34,3 +34,24 @@ def test_synthetic_is_seeded_and_block_stable():
     c = synth.voxels("c0", 10)
     assert c["data"].shape == (10, 69)
     assert np.all(c["f"].sum(1) < 1) and np.all(c["f"][:, 0] >= 0.3)
+
+
+def test_bench_refuses_more_than_one_gpu():
+    import os
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 2 and "one GPU per process" in r.stderr
+
+
+def test_bench_helpers():
+    import os
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    sys.path.insert(0, root)
+    import bench
+    # the flop model of the default workload: 82,944 per voxel and sweep (model 2, K=3, N=288)
+    assert bench.flops_per_jump(2, 3, 288) == 82944
+    assert bench.SG_DEFAULTS["nburn_short"] == 50 and bench.SG_DEFAULTS_C4["nburn_short"] == 200
