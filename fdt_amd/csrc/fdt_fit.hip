@@ -57,6 +57,17 @@ struct Mdl {
   T dfx[K], dfx0, ip0;
   int G;
   T Rv, invR;
+  // exp / log of the model and Jacobian rows: the hardware exp2 / log2 with one
+  // scaling in the FP32 production path (a few ulp, below the FP32 noise of
+  // the fit), the libm functions in the FP64 parity path
+  static FDT_DEV T ex(T x) {
+    if constexpr (std::is_same<T, float>::value) return __expf(x);
+    else return exp(x);
+  }
+  static FDT_DEV T lg(T x) {
+    if constexpr (std::is_same<T, float>::value) return __builtin_amdgcn_logf(x) * 0.6931471805599453f;
+    else return log(x);
+  }
 
   FDT_DEV void prep(const T* p) {
 #pragma unroll
@@ -127,11 +138,11 @@ struct Mdl {
   }
 
   FDT_DEV T aniso(T b, T dp) const {
-    if constexpr (KIND != KIND_MULTI) return exp(-b * d * dp * dp);
+    if constexpr (KIND != KIND_MULTI) return ex(-b * d * dp * dp);
     else {
-      if (G == 1) return exp(-b * a * bb * dp * dp);
-      if (G == 2) return exp(-b * T(3) * a * bb * invR * ((T(1) - Rv) * dp * dp + Rv));
-      return exp(-a * log(T(1) + b * bb * (dp * dp)));
+      if (G == 1) return ex(-b * a * bb * dp * dp);
+      if (G == 2) return ex(-b * T(3) * a * bb * invR * ((T(1) - Rv) * dp * dp + Rv));
+      return ex(-a * lg(T(1) + b * bb * (dp * dp)));
     }
   }
 
@@ -143,12 +154,12 @@ struct Mdl {
       err += fs[k] * aniso(b, dp);
     }
     if constexpr (KIND == KIND_MULTI) {
-      T iso = exp(-a * log(T(1) + b * bb));
+      T iso = ex(-a * lg(T(1) + b * bb));
       T S0 = fabs(p[0]);
       if constexpr (F0) return S0 * (tf0 + ((T(1) - sumf - tf0) * iso + err));
       else return S0 * ((T(1) - sumf) * iso + err);
     } else {
-      T iso = exp(-b * d);
+      T iso = ex(-b * d);
       if constexpr (F0) return p[0] * ((tf0 + (T(1) - sumf - tf0) * iso) + err);
       else return p[0] * ((T(1) - sumf) * iso + err);
     }
@@ -162,12 +173,12 @@ struct Mdl {
     T sig = T(0);
     if constexpr (KIND == KIND_SINGLE) {
       T S0 = p[0], sg1 = p[1] > T(0) ? T(1) : T(-1);
-      T iso = exp(-b * d), iso_d = -b * iso;
+      T iso = ex(-b * d), iso_d = -b * iso;
 #pragma unroll
       for (int k = 0; k < K; ++k) {
         int kk = 2 + 3 * k;
         T dp = gx * x0[k] + gy * x1[k] + gz * x2[k];
-        T e = exp(-b * d * dp * dp);
+        T e = ex(-b * d * dp * dp);
         T dp1t = cth[k] * (gx * cph[k] + gy * sph[k]) - gz * sth[k];
         T dp1p = sth[k] * (-gx * sph[k] + gy * cph[k]);
         sig += fs[k] * e;
@@ -188,12 +199,12 @@ struct Mdl {
       J0h = Jg[0];
     } else if constexpr (KIND == KIND_SINGLE_C) {
       T S0 = p[0];
-      T iso = exp(-b * d);
+      T iso = ex(-b * d);
       T A[K], dps[K];
 #pragma unroll
       for (int k = 0; k < K; ++k) {
         dps[k] = gx * x0[k] + gy * x1[k] + gz * x2[k];
-        A[k] = exp(-b * d * dps[k] * dps[k]);
+        A[k] = ex(-b * d * dps[k] * dps[k]);
       }
 #pragma unroll
       for (int k = 0; k < K; ++k) {
@@ -211,7 +222,7 @@ struct Mdl {
         Jg[kk + 1] = S0 * fs[k] * (T(-2) * b * d * dp * dp1t * A[k]);
         Jg[kk + 2] = S0 * fs[k] * (T(-2) * b * d * dp * dp1p * A[k]);
       }
-      T isol = T(-2) * b * lam * exp(-b * lam * lam);
+      T isol = T(-2) * b * lam * ex(-b * lam * lam);
       if constexpr (F0) {
         Jg[NP - 1] = S0 * (T(1) - iso) * sin(T(2) * p[NP - 1]) * part;
         sig = S0 * ((tf0 + (T(1) - sumf - tf0) * iso) + sig);
@@ -225,8 +236,8 @@ struct Mdl {
     } else {
       T S0a = fabs(p[0]);
       T sg1 = p[1] > T(0) ? T(1) : T(-1), sg2 = p[2] > T(0) ? T(1) : T(-1);
-      T la = log(T(1) + b * bb);
-      T iso = exp(-a * la);
+      T la = lg(T(1) + b * bb);
+      T iso = ex(-a * la);
       T iso_a = -la * iso;
       T iso_b = -a * b / (T(1) + b * bb) * iso;
 #pragma unroll
@@ -237,18 +248,18 @@ struct Mdl {
         T dp1p = sth[k] * (-gx * sph[k] + gy * cph[k]);
         T an, an_a, an_b, an_th, an_ph;
         if (G == 1) {
-          an = exp(-b * a * bb * dp * dp);
+          an = ex(-b * a * bb * dp * dp);
           an_a = -b * bb * dp * dp * an;
           an_b = -b * a * dp * dp * an;
           an_th = T(-2) * b * a * bb * dp * dp1t * an;
           an_ph = T(-2) * b * a * bb * dp * dp1p * an;
         } else if (G == 2) {
           T t = (T(1) - Rv) * dp * dp + Rv;
-          an = exp(-b * T(3) * a * bb * invR * t);
+          an = ex(-b * T(3) * a * bb * invR * t);
           T qa = b * T(3) * bb * invR * t;
-          an_a = -qa * exp(-qa * a);
+          an_a = -qa * ex(-qa * a);
           T qb = b * T(3) * a * invR * t;
-          an_b = -qb * exp(-qb * bb);
+          an_b = -qb * ex(-qb * bb);
           // the reference's G=2 angle derivative carries no factor (g.x)
           // (CUDA/PVM_multi.cu:77-78, :95-96); kept, it decides where LM stops
           an_th = -(T(2) * b * T(3) * a * bb * invR * (T(1) - Rv) * dp1t) * an;
@@ -256,8 +267,8 @@ struct Mdl {
         } else {
           T den = T(1) + b * (dp * dp) * bb;
           T rden = T(1) / den;
-          T l1 = log(den);
-          an = exp(-a * l1);
+          T l1 = lg(den);
+          an = ex(-a * l1);
           an_a = -l1 * an;
           an_b = -a * b * (dp * dp) * rden * an;
           T c = -a * bb * b * rden * an;
