@@ -64,6 +64,10 @@ struct Mdl {
     if constexpr (std::is_same<T, float>::value) return __expf(x);
     else return exp(x);
   }
+  static FDT_DEV T rc(T x) {  // 1/x: hardware reciprocal (1 ulp) in FP32
+    if constexpr (std::is_same<T, float>::value) return __builtin_amdgcn_rcpf(x);
+    else return T(1) / x;
+  }
   static FDT_DEV T lg(T x) {
     if constexpr (std::is_same<T, float>::value) return __builtin_amdgcn_logf(x) * 0.6931471805599453f;
     else return log(x);
@@ -239,7 +243,7 @@ struct Mdl {
       T la = lg(T(1) + b * bb);
       T iso = ex(-a * la);
       T iso_a = -la * iso;
-      T iso_b = -a * b / (T(1) + b * bb) * iso;
+      T iso_b = -a * b * rc(T(1) + b * bb) * iso;
 #pragma unroll
       for (int k = 0; k < K; ++k) {
         int kk = 3 + 3 * k;
@@ -266,7 +270,7 @@ struct Mdl {
           an_ph = -(T(2) * b * T(3) * a * bb * invR * (T(1) - Rv) * dp1p) * an;
         } else {
           T den = T(1) + b * (dp * dp) * bb;
-          T rden = T(1) / den;
+          T rden = rc(den);
           T l1 = lg(den);
           an = ex(-a * l1);
           an_a = -l1 * an;
