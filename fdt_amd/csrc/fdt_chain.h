@@ -421,10 +421,10 @@ struct Chain {
     }
   }
   // all stick signals and/or iso change: d (:924-985), d_std (:987-1051), R
-  // (:1053-1122).  Neither the candidate signals nor the candidate mixture are
-  // kept: the row sum is accumulated on the fly and everything is re-evaluated
-  // when the move is accepted, which keeps the live registers to the chain
-  // state.
+  // (:1053-1122).  The row sum of the likelihood change is accumulated on the
+  // fly; the candidate stick signals go to the wave's LDS block (mix_new) and
+  // an accepted move copies them back (store_new), re-evaluating only the
+  // isotropic signal, so the live registers stay the chain state.
   template <int SL>
   FDT_DEV void prop_global(float n, float lu) {
     const float d = v<SL_D>(), dstd = v<SL_DSTD>(), Rr = v<SL_R>();
