@@ -5,6 +5,16 @@
 
 namespace fdt {
 
+// XCD-aware tile order.  The dispatcher deals workgroups round-robin over the
+// 8 XCDs, each with its own L2; renumbered this way each XCD runs one
+// contiguous range of tiles, so the 64-byte record runs of neighbouring
+// workgroups (16 voxels x 4 B per sample and quantity) meet in one L2 and
+// leave it as whole lines.  A bijection of [0, nb) for every nb.
+__device__ inline int xcd_tile(int b, int nb) {
+  const int q = nb >> 3, r = nb & 7, x = b & 7, l = b >> 3;
+  return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + l;
+}
+
 template <int K, int VPL, bool RIC, bool GN>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VPL <= 18 ? 2 : 1, 2))) void mcmc_kernel(McmcArgs a) {
   __shared__ float4 sacq[16 * VPL];
@@ -17,7 +27,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VPL <= 18 ?
   const ChainLds L{sacq, sba, GN ? nullptr : ssh, sshb, siso, sbp, sgd};
   stage_acq<VPL>(a, L);
   __syncthreads();
-  const int i = blockIdx.x * 16 + (threadIdx.x >> 4);
+  const int i = xcd_tile(blockIdx.x, gridDim.x) * 16 + (threadIdx.x >> 4);
   if (i < a.nvox) run_chain<K, VPL, RIC, GN>(a, a.list ? a.list[i] : i, L);
 }
 
