@@ -335,6 +335,75 @@ def bench_gradnonlin(args):
     print(json.dumps(line), flush=True)
 
 
+def bench_spatial(args):
+    """The CPU xfibres initialisation (cpu_init): single-shell model 1, nf=2
+    voxels on a slab of the brain-shaped 145x174x145 mask, run with neighbour
+    initialisation (each chain started from the best finished neighbour, in
+    wavefronts x + 2y + 4z), with the tensor start (--nospat), and with the
+    xfibres_gpu constrained nonlinear fit on the same voxels."""
+    import fdt_amd as F
+    from fdt_amd import synth
+    from fdt_amd.surrogate import dyad_errors
+
+    nx, ny, nz = synth.C4_GRID
+    pos = synth.c4_positions().astype(np.int64)
+    zc = nz // 2
+    sel = pos[np.abs(pos[:, 2] - zc) < args.slab / 2.0]
+    gidx = np.sort(sel[:, 0] + nx * (sel[:, 1] + ny * sel[:, 2]))
+    if args.nvox:
+        gidx = gidx[:args.nvox]
+    nvox = len(gidx)
+    v = synth.voxels("c0", nvox)
+    N = v["bvals"].size
+    x_, y_, z_ = gidx % nx, (gidx // nx) % ny, gidx // (nx * ny)
+    nwave = len(np.unique(x_ + 2 * y_ + 4 * z_))
+    opts = dict(nfibres=2, model=1, nburn=1000, njumps=1250, sampleevery=25)
+
+    def run(**kw):
+        x = F.Xfibres(**opts, **kw)
+        try:
+            x.set_acquisition(v["bvals"], v["bvecs"])
+            x.load_data(v["data"])
+            x.set_grid((nx, ny, nz), gidx)
+            for _ in range(args.warmup):
+                x.run(0)
+                x.sync()
+            ms = []
+            t = time.perf_counter()
+            for _ in range(args.steps):
+                x.run(0)
+                x.sync()
+                ms.append(x.timing())
+            step = (time.perf_counter() - t) / args.steps
+            out = x.outputs(raw_scalars=False)
+        finally:
+            x.close()
+        e = dyad_errors(out["dyads"], out["mean_f"], v["dirs"], v["f"])
+        return dict(rate=nvox / step, ms_step=step * 1e3, ms_fit=float(np.mean([m["ms_fit"] for m in ms])),
+                    ms_mcmc=float(np.mean([m["ms_mcmc"] for m in ms])), dyad=float(np.nanmedian(e)),
+                    unmatched=float(np.mean(np.isnan(e))))
+
+    sp = run(cpu_init=1)
+    ts = run(cpu_init=1, nospat=1)
+    nl = run(cnonlin=1)
+    line = {
+        "metric": "voxels/s, 2-fibre single-shell xfibres with the CPU xfibres neighbour initialisation",
+        "value": sp["rate"], "unit": "voxels/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": sp["ms_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32", "data": "synthetic (seeded Philox), no published dataset",
+        "config": {"workload": f"single-shell model 1, nf=2, N={N}, {nvox} voxels: {args.slab} central slices "
+                               f"of the brain-shaped {nx}x{ny}x{nz} mask, bi=1000 nj=1250 se=25, cpu_init "
+                               "(neighbour initialisation)",
+                   "wavefronts": nwave, "voxels_per_wavefront": nvox / nwave,
+                   "ms_per_wavefront": sp["ms_mcmc"] / nwave, "ms_chains": sp["ms_mcmc"]},
+        "tensor_start": {"voxels_per_s": ts["rate"], "ms_start": ts["ms_fit"], "ms_mcmc": ts["ms_mcmc"]},
+        "nonlinear_fit": {"voxels_per_s": nl["rate"], "ms_fit": nl["ms_fit"], "ms_mcmc": nl["ms_mcmc"]},
+        "dyad_err_median_deg": {"neighbour": sp["dyad"], "tensor": ts["dyad"], "nonlinear": nl["dyad"]},
+        "unmatched_fibres": {"neighbour": sp["unmatched"], "tensor": ts["unmatched"], "nonlinear": nl["unmatched"]},
+    }
+    print(json.dumps(line), flush=True)
+
+
 def _write_nii(path, arr, pixdim=(2.0, 2.0, 2.0)):
     """uncompressed single-file NIfTI-1, float32 or uint8, arr indexed [x, y, z(, t)]"""
     import struct
@@ -482,12 +551,14 @@ def main():
     ap.add_argument("--nvox", type=int, default=0, help="override the voxel count (default: the full mask)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--workload", default="c1", choices=["c1", "c2", "c3", "c4", "gradnonlin", "pipeline"],
+    ap.add_argument("--workload", default="c1", choices=["c1", "c2", "c3", "c4", "gradnonlin", "pipeline", "spatial"],
                     help="c1: the multi-shell 3-fibre volume (default); c2: the mixed-acquisition queue; "
                          "c3: train the learned start and compare it with the plain path; c4: the same for "
                          "model 3 + Rician on the 904,550-voxel volume; gradnonlin: the multi-shell volume "
                          "with per-voxel gradient deviations; pipeline: the bedpostx_gpu pipeline from NIfTI "
-                         "files to the .bedpostX images")
+                         "files to the .bedpostX images; spatial: the CPU xfibres neighbour initialisation "
+                         "on a slab of the brain-shaped mask")
+    ap.add_argument("--slab", type=int, default=16, help="spatial: central z slices of the mask")
     ap.add_argument("--oracle-voxels", type=int, default=64)
     ap.add_argument("--train-voxels", type=int, default=2097152)
     ap.add_argument("--epochs", type=int, default=10)
@@ -512,6 +583,9 @@ def main():
         return
     if args.workload == "pipeline":
         bench_pipeline(args)
+        return
+    if args.workload == "spatial":
+        bench_spatial(args)
         return
 
     import fdt_amd as F

@@ -30,7 +30,7 @@ EXPORTS = ["fdt_default_opts", "fdt_create", "fdt_destroy", "fdt_last_error", "f
            "fdt_surrogate_create", "fdt_surrogate_destroy", "fdt_surrogate_last_error", "fdt_surrogate_num_weights",
            "fdt_surrogate_load", "fdt_surrogate_save", "fdt_surrogate_set_training_data", "fdt_surrogate_gradients",
            "fdt_surrogate_train_step", "fdt_surrogate_forward", "fdt_surrogate_infer", "fdt_run_surrogate", "fdt_get_fallback_flags",
-           "fdt_predict"]
+           "fdt_predict", "fdt_set_grid", "fdt_get_spatial_state"]
 
 
 class Opts(C.Structure):
@@ -39,7 +39,8 @@ class Opts(C.Structure):
         ("sampleevery", C.c_int), ("updateproposalevery", C.c_int), ("seed", C.c_uint64), ("no_ard", C.c_int),
         ("all_ard", C.c_int), ("nonlin", C.c_int), ("cnonlin", C.c_int), ("rician", C.c_int), ("f0", C.c_int),
         ("ardf0", C.c_int), ("R_prior_mean", C.c_float), ("R_prior_std", C.c_float), ("R_prior_fudge", C.c_float),
-        ("lm_fp64", C.c_int), ("reserved", C.c_int * 7),
+        ("lm_fp64", C.c_int), ("cpu_init", C.c_int), ("nospat", C.c_int), ("spatial_serial", C.c_int),
+        ("reserved", C.c_int * 4),
     ]
 
 
@@ -95,6 +96,8 @@ def lib():
         L.fdt_finish.argtypes = [vp]
         L.fdt_get_outputs.argtypes = [vp, C.POINTER(Outputs)]
         L.fdt_run.argtypes = [vp, C.c_uint64]
+        L.fdt_set_grid.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_longlong)]
+        L.fdt_get_spatial_state.argtypes = [vp, ip, _FP, _FP, _FP, C.POINTER(C.c_double)]
         L.fdt_sync.argtypes = [vp]
         L.fdt_get_timing.argtypes = [vp, C.POINTER(Timing)]
         L.fdt_set_mcmc_tile.argtypes = [vp, C.c_int]
@@ -261,6 +264,23 @@ class Xfibres:
 
     def sync(self):
         self._chk(self.L.fdt_sync(self.h))
+
+    def set_grid(self, dims, gidx):
+        """grid (nx, ny, nz) and each loaded voxel's index x + nx (y + ny z), increasing (mask order)"""
+        g = np.ascontiguousarray(gidx, dtype=np.int64)
+        self._chk(self.L.fdt_set_grid(self.h, int(dims[0]), int(dims[1]), int(dims[2]),
+                                      g.ctypes.data_as(C.POINTER(C.c_longlong))))
+
+    def spatial_state(self):
+        """chain-order state a neighbour-initialised run left: been, mean_f [K][nvox], mean_d, mean_S0, dyad [K][3][nvox]"""
+        K, nv = self.o.nfibres, self.nvox
+        st = dict(been=np.zeros(nv, np.int32), mean_f=np.zeros((K, nv), np.float32), mean_d=np.zeros(nv, np.float32),
+                  mean_S0=np.zeros(nv, np.float32), dyad=np.zeros((K, 3, nv), np.float64))
+        self._chk(self.L.fdt_get_spatial_state(self.h, st["been"].ctypes.data_as(C.POINTER(C.c_int)),
+                                               st["mean_f"].ctypes.data_as(_FP), st["mean_d"].ctypes.data_as(_FP),
+                                               st["mean_S0"].ctypes.data_as(_FP),
+                                               st["dyad"].ctypes.data_as(C.POINTER(C.c_double))))
+        return st
 
     def timing(self):
         t = Timing()

@@ -134,8 +134,10 @@ int main(int argc, char* argv[]) {
   }
   std::vector<std::string> xargs = {"bedpostx_gpu", "--nf=" + std::to_string(nfibres), "--fudge=" + fudge,
                                     "--bi=" + std::to_string(burnin), "--nj=" + std::to_string(njumps),
-                                    "--se=" + std::to_string(sampleevery), "--model=" + std::to_string(model),
-                                    "--cnonlinear"};
+                                    "--se=" + std::to_string(sampleevery), "--model=" + std::to_string(model)};
+  // --cpuinit (the CPU xfibres starts) replaces the script's constrained nonlinear fit; an explicit
+  // --cnonlinear or --nonlinear among the options still selects the fit
+  if (std::find(other.begin(), other.end(), "--cpuinit") == other.end()) xargs.push_back("--cnonlinear");
   xargs.insert(xargs.end(), other.begin(), other.end());
   std::vector<char*> xv;
   for (auto& s : xargs) xv.push_back(&s[0]);
@@ -232,6 +234,10 @@ int main(int argc, char* argv[]) {
   if (fdt_set_acquisition(ctx, N, fbvals.data(), fbvecs.data())) return fail("acquisition");
   if (fdt_load_data(ctx, dm.data(), FDT_LAYOUT_VOX_MAJOR, nvox)) return fail("data");
   if (gflag && fdt_load_grad_dev(ctx, gm.data(), FDT_LAYOUT_VOX_MAJOR, nvox)) return fail("grad_dev");
+  if (c.o.cpu_init) {  // the mask grid, for the neighbour initialisation of the CPU xfibres
+    const std::vector<long long> gidx(idx.begin(), idx.end());
+    if (fdt_set_grid(ctx, mask.dim[1], std::max(mask.dim[2], 1), std::max(mask.dim[3], 1), gidx.data())) return fail("grid");
+  }
   if (fdt_run(ctx, 0) || fdt_sync(ctx)) return fail("run");
   const fdt_opts& o = c.o;
   const int K = o.nfibres, ns = o.njumps / o.sampleevery;

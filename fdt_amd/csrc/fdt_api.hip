@@ -46,6 +46,8 @@ hipError_t fdt_launch_sg_loss(const float* O, const float* T, int M, uint16_t* d
 hipError_t fdt_launch_sg_decode(const SgDecodeArgs*, hipStream_t);
 hipError_t fdt_launch_sg_fallback(const SgFallbackArgs*, hipStream_t);
 hipError_t fdt_launch_finish(const FinishArgs*, hipStream_t);
+hipError_t fdt_launch_cpu_start(const CpuStartArgs*, hipStream_t);
+hipError_t fdt_launch_spatial_stats(const SpatialStatsArgs*, hipStream_t);
 hipError_t fdt_launch_transpose(const float*, float*, int, int, hipStream_t);
 hipError_t fdt_launch_pred(const PredArgs*, hipStream_t);
 hipError_t fdt_launch_grad_acq(const float4* base, int N, int stride, const float* grad, int nvox, float4* out,
@@ -201,6 +203,15 @@ struct fdt_ctx {
   int grad_nvox = 0, acqv_stride = 0;
   DevBuf<float> d_grad;
   DevBuf<float4> d_acqv;
+  // the voxel grid (fdt_set_grid) and the per-voxel state of the CPU xfibres
+  // neighbour initialisation, in chain fibre order
+  int gnx = 0, gny = 0, gnz = 0;
+  std::vector<long long> h_gidx;
+  DevBuf<long long> d_gidx;
+  DevBuf<int> d_vol2mat, d_been, d_wlist;
+  DevBuf<float> sp_f, sp_d, sp_S0;
+  DevBuf<double> sp_dyad;
+  bool have_spatial = false;
 };
 
 static int set_err(fdt_ctx* c, int code, const char* fmt, ...) {
@@ -361,7 +372,7 @@ int fdt_load_data(fdt_ctx* c, const float* data, int layout, int nvox) {
   c->d_stage.release();
   c->nvox = nvox;
   c->acqv_valid = false;
-  c->have_params = c->have_samples = c->have_finish = false;
+  c->have_params = c->have_samples = c->have_finish = c->have_spatial = false;
   return FDT_OK;
 }
 
@@ -474,6 +485,61 @@ static LmArgs base_lm_args(fdt_ctx* c) {
   return a;
 }
 
+// The CPU xfibres initialisation in force (xfibresVoxelManager::initialise,
+// xfibres.cc:517-541): 0 the xfibres_gpu fit (also for --rician, which the CPU
+// path always fits nonlinearly, and for --nonlinear / --cnonlinear), 1 the
+// tensor start (--nospat), 2 the best finished neighbour, else the tensor.
+static int cpu_mode(const fdt_opts& o) {
+  if (!o.cpu_init || o.rician || o.nonlin || o.cnonlin) return 0;
+  return o.nospat ? 1 : 2;
+}
+
+static CpuStartArgs cpu_start_args(fdt_ctx* c) {
+  CpuStartArgs a{};
+  a.data = c->d_data.p;
+  a.N = c->N;
+  a.nvox = c->nvox;
+  a.pinvA = c->d_pinv.p;
+  a.acqv = c->gn ? c->d_acqv.p : nullptr;
+  a.acqv_stride = c->acqv_stride;
+  a.model = c->o.model;
+  a.K = c->o.nfibres;
+  a.np = np_of(c->o);
+  a.f0 = c->o.f0;
+  a.params = c->d_params.p;
+  return a;
+}
+
+static int cpu_mode_check(fdt_ctx* c, int mode) {
+  if (c->o.model == 3)
+    return set_err(c, FDT_EUNSUP, "model 3 without nonlinear fitting: initialise_tensor sets no d for model 3 "
+                                  "(xfibres.cc:548-557), which leaves the chain at d = 0");
+  if (mode == 2 && c->o.model != 1)
+    return set_err(c, FDT_EUNSUP, "neighbour initialisation for model %d: neighbour_initialise sets no d_std "
+                                  "(xfibres.cc:338-340), which leaves it at 0; use nospat for the tensor start",
+                   c->o.model);
+  return FDT_OK;
+}
+
+// tensor start of every voxel (initialise_tensor, xfibres.cc:546-572)
+static int tensor_start_async(fdt_ctx* c) {
+  if (int rc = cpu_mode_check(c, 1)) return rc;
+  size_t nv = std::max(c->nvox, 1);
+  HIPCHK(c, c->d_params.ensure(nv * np_of(c->o)));
+  HIPCHK(c, c->d_tau.ensure(nv));
+  if (int rc = ensure_acqv(c)) return rc;
+  if (!c->quiet) HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
+  HIPCHK(c, fdt_launch_zero_ints(reinterpret_cast<int*>(c->d_tau.p), c->nvox, c->stream));  // no tau: Gaussian noise
+  CpuStartArgs a = cpu_start_args(c);
+  a.count = c->nvox;
+  HIPCHK(c, fdt_launch_cpu_start(&a, c->stream));
+  if (!c->quiet) HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
+  c->timed_fit = !c->quiet;
+  c->have_params = true;
+  c->have_samples = c->have_finish = false;
+  return FDT_OK;
+}
+
 // The fit() pipeline of CUDA/xfibres_gpu.cu:173-348 as a chain of launches
 // on the context stream: stage 1 (PVM_single, or PVM_single_c for model 1
 // without --nonlinear), PVM_multi for model >= 2, the --f0 refit of
@@ -482,6 +548,12 @@ static LmArgs base_lm_args(fdt_ctx* c) {
 static int fit_async(fdt_ctx* c) {
   if (c->N <= 0) return set_err(c, FDT_ESTATE, "no acquisition");
   const fdt_opts& o = c->o;
+  if (const int mode = cpu_mode(o)) {
+    if (mode == 2)
+      return set_err(c, FDT_ESTATE, "neighbour initialisation runs inside fdt_run (each start needs the chains of "
+                                    "the neighbours before it); set nospat for the tensor start");
+    return tensor_start_async(c);
+  }
   const int K = o.nfibres, fp64 = o.lm_fp64 ? 1 : 0, f0 = o.f0 ? 1 : 0;
   const int np = np_of(o), np1 = 2 + 3 * K + f0;
   size_t nv = std::max(c->nvox, 1);
@@ -778,6 +850,94 @@ int fdt_mcmc(fdt_ctx* c, uint64_t vox_offset) {
   return FDT_OK;
 }
 
+// The voxel loop of the CPU xfibres with neighbour initialisation
+// (xfibres.cc:898-921): start, chain and finish of each voxel in mask order,
+// each start reading what earlier neighbours left (spatial_stats_kernel).  On
+// the device the voxels run in wavefronts t = x + 2y + 4z: a neighbour that
+// precedes a voxel in mask order (dz = -1; dz = 0, dy = -1; dz = dy = 0,
+// dx = -1) has a smaller t, one that follows it a larger t, so every start
+// sees exactly the neighbours the CPU loop has finished, and no two voxels of
+// one wavefront are neighbours.  spatial_serial runs one voxel per step
+// instead, the CPU's own order.  Three launches per wavefront.
+static int spatial_async(fdt_ctx* c, uint64_t vox_offset) {
+  if (c->N <= 0) return set_err(c, FDT_ESTATE, "no acquisition");
+  if (int rc = cpu_mode_check(c, 2)) return rc;
+  const fdt_opts& o = c->o;
+  const int nv = c->nvox, K = o.nfibres, ns = nsamples_of(o);
+  if (nv > 0 && c->h_gidx.size() != (size_t)nv)
+    return set_err(c, FDT_ESTATE, "neighbour initialisation needs the voxel grid: call fdt_set_grid after fdt_load_data");
+  if (ns <= 0) return set_err(c, FDT_EINVAL, "neighbour initialisation needs recorded samples (njumps >= sampleevery)");
+  size_t nvs = std::max(nv, 1);
+  HIPCHK(c, c->d_params.ensure(nvs * np_of(o)));
+  HIPCHK(c, c->d_tau.ensure(nvs));
+  HIPCHK(c, c->d_been.ensure(nvs));
+  HIPCHK(c, c->d_wlist.ensure(nvs));
+  HIPCHK(c, c->sp_f.ensure(nvs * K));
+  HIPCHK(c, c->sp_d.ensure(nvs));
+  HIPCHK(c, c->sp_S0.ensure(nvs));
+  HIPCHK(c, c->sp_dyad.ensure(nvs * K * 3));
+  HIPCHK(c, fdt_launch_zero_ints(c->d_been.p, nv, c->stream));
+  HIPCHK(c, fdt_launch_zero_ints(reinterpret_cast<int*>(c->d_tau.p), nv, c->stream));
+  c->have_params = true;
+  McmcArgs a;
+  mcmc_launch_fn fn = nullptr;
+  if (int rc = mcmc_prepare(c, vox_offset, &a, &fn)) return rc;
+  // wavefront lists (counting sort by t, mask order inside a wavefront)
+  std::vector<int> order(nv), first;
+  if (o.spatial_serial) {
+    for (int v = 0; v < nv; ++v) order[v] = v;
+    for (int v = 0; v <= nv; ++v) first.push_back(v);
+  } else if (nv > 0) {
+    const long long nx = c->gnx, ny = c->gny;
+    const int tmax = (int)((c->gnx - 1) + 2LL * (c->gny - 1) + 4LL * (c->gnz - 1));
+    std::vector<int> t(nv), start(tmax + 2, 0);
+    for (int v = 0; v < nv; ++v) {
+      const long long g = c->h_gidx[v];
+      t[v] = (int)(g % nx + 2 * ((g / nx) % ny) + 4 * (g / (nx * ny)));
+      ++start[t[v] + 1];
+    }
+    for (int i = 0; i <= tmax; ++i) start[i + 1] += start[i];
+    std::vector<int> fill(start.begin(), start.end() - 1);
+    for (int v = 0; v < nv; ++v) order[fill[t[v]]++] = v;
+    for (int i = 0; i <= tmax; ++i)
+      if (start[i + 1] > start[i]) first.push_back(start[i]);
+    first.push_back(nv);
+  }
+  HIPCHK(c, hipStreamSynchronize(c->stream));  // an earlier run may still read the lists
+  if (nv > 0) HIPCHK(c, hipMemcpy(c->d_wlist.p, order.data(), (size_t)nv * sizeof(int), hipMemcpyHostToDevice));
+  CpuStartArgs st = cpu_start_args(c);
+  st.vol2mat = c->d_vol2mat.p;
+  st.gidx = c->d_gidx.p;
+  st.nx = c->gnx; st.ny = c->gny; st.nz = c->gnz;
+  st.been = c->d_been.p;
+  st.mean_f = c->sp_f.p; st.mean_d = c->sp_d.p; st.mean_S0 = c->sp_S0.p; st.dyad = c->sp_dyad.p;
+  SpatialStatsArgs ss{};
+  ss.K = K; ss.ns = ns; ss.vstride = nv;
+  ss.th = c->s_th.p; ss.ph = c->s_ph.p; ss.f = c->s_f.p; ss.S0 = c->s_S0.p; ss.d = c->s_d.p;
+  ss.been = c->d_been.p;
+  ss.mean_f = c->sp_f.p; ss.mean_d = c->sp_d.p; ss.mean_S0 = c->sp_S0.p; ss.dyad = c->sp_dyad.p;
+  if (!c->quiet) HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
+  for (size_t g = 0; g + 1 < first.size(); ++g) {
+    const int* L = c->d_wlist.p + first[g];
+    const int n = first[g + 1] - first[g];
+    st.list = L; st.count = n;
+    HIPCHK(c, fdt_launch_cpu_start(&st, c->stream));
+    McmcArgs b = a;
+    b.list = L;
+    b.nvox = n;
+    HIPCHK(c, fn(&b, K, 0, c->stream));
+    ss.list = L; ss.count = n;
+    HIPCHK(c, fdt_launch_spatial_stats(&ss, c->stream));
+  }
+  if (!c->quiet) HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
+  c->timed_fit = false;
+  c->timed_mcmc = !c->quiet;
+  c->have_samples = true;
+  c->have_finish = false;
+  c->have_spatial = true;
+  return FDT_OK;
+}
+
 static int finish_async(fdt_ctx* c) {
   if (!c->have_samples) return set_err(c, FDT_ESTATE, "no samples: call fdt_mcmc first");
   const fdt_opts& o = c->o;
@@ -856,10 +1016,54 @@ int fdt_get_outputs(fdt_ctx* c, const fdt_outputs* out) {
 int fdt_run(fdt_ctx* c, uint64_t vox_offset) {
   if (!c) return FDT_EINVAL;
   (void)hipSetDevice(c->device);
-  int rc = fit_async(c);
-  if (!rc) rc = mcmc_async(c, vox_offset);
+  int rc;
+  if (cpu_mode(c->o) == 2) {
+    rc = spatial_async(c, vox_offset);
+  } else {
+    rc = fit_async(c);
+    if (!rc) rc = mcmc_async(c, vox_offset);
+  }
   if (!rc) rc = finish_async(c);
   return rc;
+}
+
+int fdt_set_grid(fdt_ctx* c, int nx, int ny, int nz, const long long* gidx) {
+  if (!c) return FDT_EINVAL;
+  if (nx <= 0 || ny <= 0 || nz <= 0) return set_err(c, FDT_EINVAL, "grid %d x %d x %d", nx, ny, nz);
+  const long long nxyz = (long long)nx * ny * nz;
+  if (nxyz > (1LL << 31)) return set_err(c, FDT_EINVAL, "grid of %lld voxels", nxyz);
+  if (c->nvox > 0 && !gidx) return set_err(c, FDT_EINVAL, "null voxel indices");
+  for (int v = 0; v < c->nvox; ++v) {
+    if (gidx[v] < 0 || gidx[v] >= nxyz) return set_err(c, FDT_EINVAL, "voxel %d: grid index %lld outside the grid", v, gidx[v]);
+    if (v > 0 && gidx[v] <= gidx[v - 1]) return set_err(c, FDT_EINVAL, "voxel %d: grid indices must increase (mask order)", v);
+  }
+  (void)hipSetDevice(c->device);
+  HIPCHK(c, hipStreamSynchronize(c->stream));  // a run in flight may still read the grid
+  std::vector<int> v2m((size_t)nxyz, 0);
+  for (int v = 0; v < c->nvox; ++v) v2m[(size_t)gidx[v]] = v + 1;
+  HIPCHK(c, c->d_vol2mat.ensure((size_t)nxyz));
+  HIPCHK(c, c->d_gidx.ensure(std::max(c->nvox, 1)));
+  HIPCHK(c, hipMemcpy(c->d_vol2mat.p, v2m.data(), (size_t)nxyz * sizeof(int), hipMemcpyHostToDevice));
+  if (c->nvox > 0) HIPCHK(c, hipMemcpy(c->d_gidx.p, gidx, (size_t)c->nvox * sizeof(long long), hipMemcpyHostToDevice));
+  c->gnx = nx; c->gny = ny; c->gnz = nz;
+  c->h_gidx.assign(gidx, gidx + c->nvox);
+  return FDT_OK;
+}
+
+int fdt_get_spatial_state(fdt_ctx* c, int* been, float* mean_f, float* mean_d, float* mean_S0, double* dyad) {
+  if (!c) return FDT_EINVAL;
+  if (!c->have_spatial) return set_err(c, FDT_ESTATE, "no neighbour-initialised run");
+  (void)hipSetDevice(c->device);
+  const size_t nv = c->nvox, K = c->o.nfibres;
+  if (nv) {
+    if (been) HIPCHK(c, hipMemcpyAsync(been, c->d_been.p, nv * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    if (mean_f) HIPCHK(c, hipMemcpyAsync(mean_f, c->sp_f.p, nv * K * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    if (mean_d) HIPCHK(c, hipMemcpyAsync(mean_d, c->sp_d.p, nv * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    if (mean_S0) HIPCHK(c, hipMemcpyAsync(mean_S0, c->sp_S0.p, nv * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    if (dyad) HIPCHK(c, hipMemcpyAsync(dyad, c->sp_dyad.p, nv * K * 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return FDT_OK;
 }
 
 int fdt_sync(fdt_ctx* c) {

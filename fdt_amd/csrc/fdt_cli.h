@@ -65,7 +65,9 @@ inline void usage() {
                "\t--Rstd\tSet the prior standard deviation for R of model 3 (default:0.03)\n"
                "\t--Rfudge\tIf set(>0), an ARD prior is used for R with the requested fudge factor\n"
                "\t--gradnonlin\tGradient Nonlinearity Tensor file\n"
-               "\t--savedyads\twrite dyads<k>J, mean_f<k>samplesJ and dispersion<k>J as well (MI355X engine extension)\n";
+               "\t--savedyads\twrite dyads<k>J, mean_f<k>samplesJ and dispersion<k>J as well (MI355X engine extension)\n"
+               "\t--cpuinit\tStart the chains as the CPU xfibres does: without --nonlinear, --cnonlinear or --rician,\n"
+               "\t\tfrom the tensor (--nospat) or the best finished neighbour (MI355X engine extension)\n";
 }
 
 // FSL Option<T> parsing: "--name=value", "--name value", "-x value"; flags take no value.
@@ -123,7 +125,8 @@ inline bool parse(int argc, char** argv, Cli& c, std::string& err) {
     else if (is({"--seed"})) { if (!as_u64(c.o.seed)) return false; }
     else if (is({"--noard"})) c.o.no_ard = 1;
     else if (is({"--allard"})) c.o.all_ard = 1;
-    else if (is({"--nospat"})) c.localinit = true;
+    else if (is({"--nospat"})) { c.localinit = true; c.o.nospat = 1; }
+    else if (is({"--cpuinit"})) c.o.cpu_init = 1;
     else if (is({"--nonlinear"})) c.o.nonlin = 1;
     else if (is({"--cnonlinear"})) c.o.cnonlin = 1;
     else if (is({"--rician"})) c.o.rician = 1;

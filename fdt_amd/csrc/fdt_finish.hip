@@ -123,6 +123,55 @@ __global__ __launch_bounds__(256) void finish_kernel(FinishArgs a) {
   if (a.mean_tau && a.taus) a.mean_tau[v] = mean_of(a.taus);
 }
 
+// What the CPU xfibres keeps of each finished voxel for the starts of its
+// neighbours (Samples::record / finish_voxel, xfibres.cc:215-303), in chain
+// fibre order (the CPU path does not sort fibres): float sums over the
+// samples divided by the sample count, the dyadic tensor summed in double
+// from float direction components, its principal eigenvector (sign: largest
+// component positive), and the count of fibres whose mean f exceeds 0.05.
+// One thread per listed voxel.
+__global__ __launch_bounds__(256) void spatial_stats_kernel(SpatialStatsArgs a) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= a.count) return;
+  const int v = a.list ? a.list[q] : q;
+  const size_t nv = (size_t)a.vstride;
+  const int ns = a.ns;
+  float sd = 0.0f, sS0 = 0.0f;
+  for (int s = 0; s < ns; ++s) {
+    sd += a.d[(size_t)s * nv + v];
+    sS0 += a.S0[(size_t)s * nv + v];
+  }
+  a.mean_d[v] = sd / (float)ns;
+  a.mean_S0[v] = sS0 / (float)ns;
+  int nfib = 0;
+  for (int k = 0; k < a.K; ++k) {
+    float sf = 0.0f;
+    double T[6] = {0, 0, 0, 0, 0, 0};
+    for (int s = 0; s < ns; ++s) {
+      const size_t is = ((size_t)k * ns + s) * nv + v;
+      const float th = a.th[is], ph = a.ph[is];
+      sf += a.f[is];
+      const double x = (double)(sinf(th) * cosf(ph)), y = (double)(sinf(th) * sinf(ph)), z = (double)cosf(th);
+      T[0] += x * x; T[1] += x * y; T[2] += x * z; T[3] += y * y; T[4] += y * z; T[5] += z * z;
+    }
+    const float mf = sf / (float)ns;
+    a.mean_f[(size_t)k * nv + v] = mf;
+    if ((double)mf > 0.05) ++nfib;
+    double A[9] = {T[0], T[1], T[2], T[1], T[3], T[4], T[2], T[4], T[5]};
+    double w[3], V[9];
+    jacobi3_sym(A, w, V);
+    int m = 0;
+    if (w[1] > w[m]) m = 1;
+    if (w[2] > w[m]) m = 2;
+    double e[3] = {V[0 * 3 + m], V[1 * 3 + m], V[2 * 3 + m]};
+    int big = 0;
+    for (int i = 1; i < 3; ++i) if (fabs(e[i]) > fabs(e[big])) big = i;
+    if (e[big] < 0) for (int i = 0; i < 3; ++i) e[i] = -e[i];
+    for (int i = 0; i < 3; ++i) a.dyad[((size_t)k * 3 + i) * nv + v] = e[i];
+  }
+  a.been[v] = nfib;
+}
+
 // [N][nvox] -> [nvox][N] through 32x33 LDS tiles (the reference transposes on
 // the host, prepare_data_gpu_FIT, CUDA/xfibres_gpu.cu:388-399)
 __global__ __launch_bounds__(256) void transpose_kernel(const float* __restrict__ in, float* __restrict__ out, int N, int nvox) {
@@ -180,6 +229,13 @@ extern "C" hipError_t fdt_launch_finish(const fdt::FinishArgs* a, hipStream_t s)
   else if (a->K == 2) hipLaunchKernelGGL(fdt::finish_kernel<2>, grid, block, 0, s, *a);
   else if (a->K == 3) hipLaunchKernelGGL(fdt::finish_kernel<3>, grid, block, 0, s, *a);
   else return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+extern "C" hipError_t fdt_launch_spatial_stats(const fdt::SpatialStatsArgs* a, hipStream_t s) {
+  if (a->count <= 0) return hipSuccess;
+  if (a->ns <= 0 || a->K < 1 || a->K > 3) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(fdt::spatial_stats_kernel, dim3((a->count + 255) / 256), dim3(256), 0, s, *a);
   return hipGetLastError();
 }
 

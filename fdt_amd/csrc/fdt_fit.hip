@@ -1112,6 +1112,88 @@ __global__ __launch_bounds__(256) void tau_kernel(TauArgs a) {
   if (rl == 0) a.tau[vox] = (float)(1.0 / (ss / (a.N - 1)));
 }
 
+// The start of a chain as the CPU xfibres sets it for Gaussian noise without
+// --nonlinear / --cnonlinear (xfibresVoxelManager::initialise, xfibres.cc:517-541):
+// the means of the best finished neighbour (Samples::neighbour_initialise,
+// :306-357) when the grid is given and one qualifies, else the tensor
+// (initialise_tensor, :546-572).  Values are rounded to float where the
+// reference holds them in float.  One 16-lane row per listed voxel; every
+// lane of the row scans the neighbourhood, lane 0 stores.
+template <bool GN>
+__global__ __launch_bounds__(256) void cpu_start_kernel(CpuStartArgs a) {
+  const int rl = __lane_id() & 15;
+  const int q = blockIdx.x * 16 + (threadIdx.x >> 4);
+  if (q >= a.count) return;
+  const int vox = a.list ? a.list[q] : q;
+  const size_t nv = (size_t)a.nvox;
+  const int K = a.K, add = a.model >= 2 ? 1 : 0;
+  // neighbour_initialise: scan x, then y, then z (outer to inner) over the
+  // 3x3x3 block; a finished voxel with at least maxfib supported fibres and a
+  // larger sum of mean f becomes the best and raises maxfib to its count.
+  // Outside the grid nothing is finished (newimage's zero extrapolation).
+  int best = -1;
+  if (a.vol2mat) {
+    const long long g = a.gidx[vox];
+    const int xx = (int)(g % a.nx), yy = (int)((g / a.nx) % a.ny), zz = (int)(g / ((long long)a.nx * a.ny));
+    int maxfib = 1;
+    float maxsf = 0.0f;
+    for (int x = xx - 1; x <= xx + 1; ++x)
+      for (int y = yy - 1; y <= yy + 1; ++y)
+        for (int z = zz - 1; z <= zz + 1; ++z) {
+          if (x < 0 || y < 0 || z < 0 || x >= a.nx || y >= a.ny || z >= a.nz) continue;
+          const int vn = a.vol2mat[((size_t)z * a.ny + y) * a.nx + x];
+          const int been = vn > 0 ? a.been[vn - 1] : 0;
+          if (been < maxfib) continue;
+          float sumf = 0.0f;
+          for (int k = 0; k < K; ++k) sumf += a.mean_f[(size_t)k * nv + (vn - 1)];
+          if (sumf > maxsf) { maxsf = sumf; maxfib = been; best = vn - 1; }
+        }
+    if (maxfib <= 1) best = -1;
+  }
+  float S0, d, dstd = 0.0f, f[3], th[3], ph[3];
+  if (best >= 0) {
+    d = a.mean_d[best];
+    S0 = a.mean_S0[best];
+    for (int k = 0; k < K; ++k) {
+      double v[3] = {a.dyad[((size_t)k * 3 + 0) * nv + best], a.dyad[((size_t)k * 3 + 1) * nv + best],
+                     a.dyad[((size_t)k * 3 + 2) * nv + best]};
+      double t, p;
+      cart2sph(v, t, p);
+      th[k] = (float)t; ph[k] = (float)p;
+      f[k] = a.mean_f[(size_t)k * nv + best];
+    }
+  } else {
+    Row R;
+    R.rl = rl; R.N = a.N; R.y = a.data + (size_t)vox * a.N;
+    R.acq = GN ? a.acqv + (size_t)vox * a.acqv_stride : nullptr;
+    Dti dti;
+    if constexpr (GN) dti = dti_row_gn(R);
+    else dti = dti_row(R, a.pinvA);
+    float D = (float)dti.md;
+    if (a.model >= 2) D = D * 2.0f;  // the mono-exponential tensor underestimates d (:554-556)
+    if (D <= 0.0f) D = 2e-3f;
+    d = D;
+    if (a.model >= 2) dstd = D;
+    S0 = (float)dti.s0;
+    double t, p;
+    cart2sph(dti.v[0], t, p);
+    th[0] = (float)t; ph[0] = (float)p;
+    f[0] = (float)dti.fa;
+    for (int k = 1; k < K; ++k) { th[k] = (float)(M_PI / 2); ph[k] = 0.0f; f[k] = 0.01f; }  // Fibre() defaults, fibre.h:103-110
+  }
+  if (rl != 0) return;
+  float* P = a.params + (size_t)vox * a.np;
+  P[0] = S0;
+  P[1] = d;
+  if (add) P[2] = dstd;
+  for (int k = 0; k < K; ++k) {
+    P[2 + add + 3 * k] = f[k];
+    P[3 + add + 3 * k] = th[k];
+    P[4 + add + 3 * k] = ph[k];
+  }
+  if (a.f0) P[a.np - 1] = 0.0f;  // Multifibre starts f0 at 0 (fibre.h:666)
+}
+
 }  // namespace fdt
 
 // One translation unit per (K, precision), compiled in parallel:
@@ -1169,3 +1251,15 @@ extern "C" hipError_t FDT_CAT3(fdt_launch_tau_k, FDT_FIT_K, FDT_FIT_FP64)(const 
   hipLaunchKernelGGL((fdt::tau_kernel<FitReal, FDT_FIT_K>), grid, block, 0, s, *a);
   return hipGetLastError();
 }
+
+#if FDT_FIT_K == 1 && FDT_FIT_FP64 == 1
+// the CPU xfibres starts (one translation unit: K is a run-time value there)
+extern "C" hipError_t fdt_launch_cpu_start(const fdt::CpuStartArgs* a, hipStream_t s) {
+  if (a->count <= 0) return hipSuccess;
+  if (a->N > fdt::MAX_N || a->K < 1 || a->K > 3) return hipErrorInvalidValue;
+  dim3 grid((a->count + 15) / 16), block(256);
+  if (a->acqv) hipLaunchKernelGGL(fdt::cpu_start_kernel<true>, grid, block, 0, s, *a);
+  else hipLaunchKernelGGL(fdt::cpu_start_kernel<false>, grid, block, 0, s, *a);
+  return hipGetLastError();
+}
+#endif

@@ -64,6 +64,40 @@ struct TauArgs {
   int acqv_stride;
 };
 
+// Chain starts as the CPU xfibres sets them without nonlinear fitting
+// (cpu_start_kernel, fdt_fit.hip): the tensor start, or the means of the best
+// finished neighbour when the grid is given.
+struct CpuStartArgs {
+  const float* data;      // voxel-major [nvox][N]
+  int N, nvox;            // nvox: voxels of the volume (stride of the per-voxel arrays)
+  const double* pinvA;    // [7][N] pseudo-inverse of the tensor design
+  const float4* acqv;     // --gradnonlin: per-voxel acquisitions [nvox][acqv_stride], or null
+  int acqv_stride;
+  const int* list;        // voxels to start: list[0..count), or 0..count-1 when null
+  int count;
+  int model, K, np, f0;   // chain parameter layout [S0, d, (dstd), (f, th, ph) x K, (f0)]
+  const int* vol2mat;     // [nz][ny][nx]: mask index + 1, 0 outside; null: tensor start only
+  const long long* gidx;  // [nvox]: grid index x + nx (y + ny z) of each voxel
+  int nx, ny, nz;
+  const int* been;        // [nvox]: fibres with mean f > 0.05 of a finished voxel, 0 before
+  const float* mean_f;    // [K][nvox], chain fibre order
+  const float* mean_d;    // [nvox]
+  const float* mean_S0;   // [nvox]
+  const double* dyad;     // [K][3][nvox]
+  float* params;          // [nvox][np] out
+};
+
+// Per-voxel means and dyads of the listed voxels' raw samples in chain fibre
+// order, and their supported-fibre count (spatial_stats_kernel, fdt_finish.hip)
+struct SpatialStatsArgs {
+  int K, ns, vstride, count;
+  const int* list;
+  const float *th, *ph, *f, *S0, *d;  // raw samples: th/ph/f [K][ns][vstride], scalars [ns][vstride]
+  int* been;
+  float *mean_f, *mean_d, *mean_S0;
+  double* dyad;
+};
+
 struct McmcArgs {
   const float* data;    // voxel-major [nvox][N]
   const float* params;  // [nvox][np] LM output

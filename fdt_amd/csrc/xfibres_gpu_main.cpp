@@ -137,6 +137,11 @@ int main(int argc, char* argv[]) {
   if (c.o.rician && !c.o.nonlin)
     std::cout << "Rician noise model requested. Non-linear parameter initialization will be performed, overriding other initialization options!" << std::endl << std::endl;
 
+  if (c.o.cpu_init && !c.o.nospat && !c.o.rician && !c.o.nonlin && !c.o.cnonlin) {
+    std::cerr << "xfibres_gpu: --cpuinit needs --nospat here: the part files carry no voxel grid for the neighbour "
+                 "initialisation (bedpostx_gpu has it)" << std::endl;
+    return EXIT_FAILURE;
+  }
   fdt_ctx* ctx = fdt_create(0, &c.o);
   if (!ctx) { std::cerr << "xfibres_gpu: cannot create the engine (options or device)" << std::endl; return EXIT_FAILURE; }
   auto fail = [&](const char* what) {

@@ -67,7 +67,14 @@ typedef struct fdt_opts {
   float R_prior_std;       /* --Rstd */
   float R_prior_fudge;     /* --Rfudge */
   int lm_fp64;             /* 0: FP32 model/Jacobian (production); 1: FP64 throughout (parity check) */
-  int reserved[7];
+  int cpu_init;            /* 1: start the chains as the CPU xfibres does (xfibres.cc:517-541): without
+                              nonlinear fitting for Gaussian noise when neither nonlin nor cnonlin is set,
+                              from the tensor (nospat) or from the best finished neighbour (needs the
+                              voxel grid, fdt_set_grid); 0: the xfibres_gpu fit (CUDA/xfibres_gpu.cu:173-348) */
+  int nospat;              /* --nospat (xfibresoptions.h:134): tensor start, no neighbour initialisation */
+  int spatial_serial;      /* neighbour initialisation one voxel per step in mask order, the CPU's own order
+                              (checks the wavefront schedule, which gives identical results); 0: wavefronts */
+  int reserved[4];
 } fdt_opts;
 
 /* Output buffers; any pointer may be NULL to skip that output. */
@@ -182,8 +189,26 @@ int fdt_finish(fdt_ctx* c);
 int fdt_get_outputs(fdt_ctx* c, const fdt_outputs* out);
 
 /* fit + mcmc + finish on the resident data, asynchronous on the context
- * stream: the per-part pipeline of xfibres_gpu(), CUDA/xfibres_gpu.cu:29-146. */
+ * stream: the per-part pipeline of xfibres_gpu(), CUDA/xfibres_gpu.cu:29-146.
+ * With cpu_init and neighbour initialisation it is the voxel loop of the CPU
+ * xfibres instead (xfibres.cc:898-921): each chain starts from the means of
+ * its best finished neighbour (Samples::neighbour_initialise, xfibres.cc:306-357),
+ * run on the device in wavefronts x + 2y + 4z, which order every voxel after
+ * the neighbours that precede it in the mask; fdt_fit is then refused. */
 int fdt_run(fdt_ctx* c, uint64_t vox_offset);
+
+/* The grid of the loaded voxels for neighbour initialisation: nx, ny, nz and
+ * each voxel's index x + nx (y + ny z), strictly increasing (mask order, the
+ * columns of volume4D::matrix(mask) that xfibres.cc:874-876 reads with
+ * vol2matrixkey / matrix2volkey).  No counterpart in the xfibres_gpu boundary. */
+int fdt_set_grid(fdt_ctx* c, int nx, int ny, int nz, const long long* gidx);
+
+/* After a neighbour-initialised fdt_run: what each finished voxel left for its
+ * neighbours (Samples::finish_voxel, xfibres.cc:249-303), in chain fibre order:
+ * been [nvox] fibres with mean f > 0.05, mean_f [K][nvox], mean_d and mean_S0
+ * [nvox], dyad [K][3][nvox] (principal eigenvector of the summed dyadic tensor,
+ * largest component positive).  NULL fields are skipped. */
+int fdt_get_spatial_state(fdt_ctx* c, int* been, float* mean_f, float* mean_d, float* mean_S0, double* dyad);
 int fdt_sync(fdt_ctx* c);
 int fdt_get_timing(fdt_ctx* c, fdt_timing* t);
 

@@ -122,3 +122,12 @@ def test_cli_gradnonlin_file_checked_before_gpu_work(subj):
     assert r.returncode == 1 and "gradnonlin file grad_short holds fewer than 9 x 10" in r.stderr
     r = run_cli(base + ["--gradnonlin=missing", "S", "0", "1", "10"], subj)
     assert r.returncode == 1 and "cannot open gradnonlin file missing" in r.stderr
+
+
+def test_cli_cpuinit_needs_nospat_without_a_grid(subj):
+    # the part files carry no voxel grid: neighbour initialisation is refused before the engine starts
+    import numpy as np
+    np.ones((8, 10)).tofile(subj / "data_0")
+    base = ["--data=data_0", "--mask=m", "-b", "bvals", "-r", "bvecs", "--logdir=L", "--forcedir"]
+    r = run_cli(base + ["--cpuinit", "S", "0", "1", "10"], subj)
+    assert r.returncode == 1 and "--cpuinit needs --nospat here" in r.stderr

@@ -105,3 +105,30 @@ def test_pipeline_end_to_end(fdt, tmp_path, grad):
     # a second run refuses to overwrite
     r2 = subprocess.run(cmd, capture_output=True, text=True, timeout=60)
     assert r2.returncode == 1 and "has already been processed" in r2.stderr
+
+
+def test_pipeline_cpu_initialisation(fdt, tmp_path):
+    """--cpuinit: the chains start as the CPU xfibres starts them (neighbour initialisation over the mask
+    grid, tensor fallback); the merged samples equal the library's neighbour-initialised run on that grid"""
+    s, v, (xs, ys, zs) = make_subject(tmp_path)
+    cmd = [PIPE, str(s), "-n", "2", "-b", "50", "-j", "100", "-s", "25", "-model", "1", "--cpuinit"]
+    print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    x = fdt.Xfibres(nfibres=2, model=1, nburn=50, njumps=100, sampleevery=25, cpu_init=1, seed=8665904)
+    try:
+        x.set_acquisition(v["bvals"], v["bvecs"])
+        x.load_data(v["data"])
+        x.set_grid(SHAPE, xs + SHAPE[0] * (ys + SHAPE[1] * zs))
+        x.run()
+        x.sync()
+        ref = x.outputs()
+        st = x.spatial_state()
+    finally:
+        x.close()
+    assert (st["been"] >= 2).any()
+    B = tmp_path / "subj.bedpostX"
+    for k in (1, 2):
+        for q in ("th", "ph", "f"):
+            img, _ = NI.read(B / f"merged_{q}{k}samples.nii.gz")
+            np.testing.assert_array_equal(img[xs, ys, zs, :], ref[q][k - 1].T)

@@ -1,0 +1,180 @@
+"""CPU xfibres chain starts on the GPU (cpu_init): the tensor start
+(--nospat, initialise_tensor xfibres.cc:546-572) against the oracle's tensor
+fit, and the neighbour initialisation (Samples::neighbour_initialise,
+xfibres.cc:306-357) in three ways:
+  * the wavefront schedule gives bit-identical results to one voxel per step
+    in mask order, the CPU loop's own order;
+  * what each finished voxel left (chain-order means, dyads, supported-fibre
+    count) equals the oracle's finish statistics of that voxel's samples;
+  * the start every voxel ran from equals the oracle's choice over the
+    neighbours finished before it, given that state.
+Tolerances: means and counts exact; dyads 1e-6 (device Jacobi against LAPACK
+on the same float direction components); angles from cart2sph 1e-6 rad."""
+import numpy as np
+import pytest
+
+from fdt_amd import synth
+from oracle import spatial_oracle as S
+
+pytestmark = pytest.mark.gpu
+
+f32 = np.float32
+
+
+def grid_volume(dims, frac, seed, name="c0"):
+    """a mask over `dims` (frac of the voxels, in mask order) with synthetic voxels of workload `name`"""
+    g = np.random.default_rng(seed)
+    n = int(np.prod(dims))
+    gidx = np.sort(g.choice(n, size=int(n * frac), replace=False)).astype(np.int64)
+    v = synth.voxels(name, len(gidx))
+    return v, gidx
+
+
+def make(fdt, v, **opts):
+    x = fdt.Xfibres(**opts)
+    x.set_acquisition(v["bvals"], v["bvecs"])
+    x.load_data(v["data"])
+    return x
+
+
+@pytest.mark.parametrize("model,K,f0", [(1, 2, False), (1, 3, True), (2, 2, False)])
+def test_tensor_start_matches_oracle(orc, fdt, model, K, f0):
+    v = synth.voxels("c0" if model == 1 else "c1", 96)
+    x = make(fdt, v, nfibres=K, model=model, f0=int(f0), cpu_init=1, nospat=1, njumps=100, nburn=50, sampleevery=10)
+    try:
+        x.fit()
+        P, tau = x.get_params()
+        x.mcmc()
+        x.finish()
+        out = x.outputs()
+    finally:
+        x.close()
+    rows = orc.dti(v["bvals"], v["bvecs"], v["data"].astype(np.float64))
+    E = np.stack([S.tensor_start(model, K, f0, r) for r in rows])
+    add = 1 if model >= 2 else 0
+    ang = [3 + add + 3 * k for k in range(K)] + [4 + add + 3 * k for k in range(K)]
+    rest = [i for i in range(E.shape[1]) if i not in ang]
+    np.testing.assert_allclose(P[:, rest], E[:, rest], rtol=2e-6, atol=0)
+    # the principal direction (float angles: within 1e-6 of the same axis)
+    P64, E64 = P.astype(np.float64), E.astype(np.float64)
+    u = lambda th, ph: np.stack([np.sin(th) * np.cos(ph), np.sin(th) * np.sin(ph), np.cos(th)], -1)
+    dots = np.abs((u(P64[:, 3 + add], P64[:, 4 + add]) * u(E64[:, 3 + add], E64[:, 4 + add])).sum(-1))
+    assert np.all(dots > 1 - 1e-6)
+    assert np.all(tau == 0)
+    assert np.all(np.isfinite(out["mean_f"])) and np.all(np.isfinite(out["mean_d"]))
+
+
+def test_tensor_start_gradnonlin(orc, fdt):
+    v = synth.gradnonlin_voxels("c0", 24)
+    x = make(fdt, v, nfibres=2, model=1, cpu_init=1, nospat=1, njumps=50, nburn=0, sampleevery=10)
+    try:
+        x.load_grad_dev(v["grad"])
+        x.fit()
+        P, _ = x.get_params()
+    finally:
+        x.close()
+    for i in range(24):
+        bv, bc = orc.correct_acq(v["bvals"], v["bvecs"], v["grad"][i].astype(np.float64))
+        r = orc.dti(bv, bc, v["data"][i:i + 1].astype(np.float64))[0]
+        e = S.tensor_start(1, 2, False, r)
+        np.testing.assert_allclose(P[i, [0, 1, 2]], e[[0, 1, 2]], rtol=2e-6)
+
+
+def spatial_run(fdt, v, dims, gidx, serial, K=2, **kw):
+    opts = dict(nfibres=K, model=1, cpu_init=1, njumps=60, nburn=40, sampleevery=6, seed=4242)
+    opts.update(kw)
+    x = make(fdt, v, spatial_serial=int(serial), **opts)
+    try:
+        x.set_grid(dims, gidx)
+        x.run()
+        x.sync()
+        out = x.outputs()
+        P, _ = x.get_params()
+        st = x.spatial_state()
+    finally:
+        x.close()
+    return out, P, st
+
+
+def test_wavefronts_equal_the_serial_order(fdt):
+    dims = (6, 5, 4)
+    v, gidx = grid_volume(dims, 0.8, 11)
+    print(f"spatial: {len(gidx)} voxels on {dims}, wavefront vs serial", flush=True)
+    a = spatial_run(fdt, v, dims, gidx, serial=False)
+    b = spatial_run(fdt, v, dims, gidx, serial=True)
+    for k in a[0]:
+        np.testing.assert_array_equal(a[0][k], b[0][k], err_msg=k)
+    np.testing.assert_array_equal(a[1], b[1])
+    for k in a[2]:
+        np.testing.assert_array_equal(a[2][k], b[2][k], err_msg=k)
+
+
+def test_state_and_starts_follow_the_cpu_rules(orc, fdt):
+    dims = (7, 6, 5)
+    v, gidx = grid_volume(dims, 0.75, 12)
+    nv, K = len(gidx), 2
+    print(f"spatial: {nv} voxels on {dims}", flush=True)
+    out, P, st = spatial_run(fdt, v, dims, gidx, serial=False)
+    # per-voxel state against the oracle's finish statistics of the voxel's own samples; the outputs hold the
+    # fibres sorted by their sum of f, so the chain order is the permutation whose statistics match
+    for i in range(nv):
+        ok = False
+        for perm in ([0, 1], [1, 0]):
+            b, mf, md, mS0, dy = S.finish_stats(out["th"][perm, :, i], out["ph"][perm, :, i], out["f"][perm, :, i],
+                                                out["d"][:, i], out["S0"][:, i])
+            if np.array_equal(mf, st["mean_f"][:, i]):
+                ok = True
+                assert b == st["been"][i] and md == st["mean_d"][i] and mS0 == st["mean_S0"][i], i
+                np.testing.assert_allclose(st["dyad"][:, :, i], dy, atol=1e-6, err_msg=str(i))
+                break
+        assert ok, f"voxel {i}: no fibre order of the outputs gives the device state"
+    # the start every voxel ran from: tensor starts from a --nospat pass over the same data
+    x = make(fdt, v, nfibres=K, model=1, cpu_init=1, nospat=1, njumps=60, nburn=40, sampleevery=6)
+    try:
+        x.fit()
+        T, _ = x.get_params()
+    finally:
+        x.close()
+    E, src = S.expected_starts(K, False, gidx, dims, st, T)
+    used = src >= 0
+    print(f"spatial: {used.mean():.3f} of voxels started from a neighbour", flush=True)
+    assert used.mean() > 0.1
+    ang = [3, 4, 6, 7]
+    rest = [0, 1, 2, 5]
+    np.testing.assert_array_equal(P[:, rest], E[:, rest])
+    np.testing.assert_allclose(P[:, ang], E[:, ang], atol=1e-6)
+    assert np.all(np.isfinite(out["mean_f"]))
+
+
+def test_errors(fdt):
+    v, gidx = grid_volume((4, 4, 4), 0.5, 13)
+    for opts, msg in [(dict(model=3, nospat=1), "model 3"), (dict(model=2), "model 2")]:
+        x = make(fdt, v, nfibres=2, cpu_init=1, **opts)
+        try:
+            x.set_grid((4, 4, 4), gidx)
+            with pytest.raises(fdt.FdtError, match=msg):
+                x.run()
+        finally:
+            x.close()
+    x = make(fdt, v, nfibres=2, model=1, cpu_init=1)
+    try:
+        with pytest.raises(fdt.FdtError, match="fdt_set_grid"):
+            x.run()
+        with pytest.raises(fdt.FdtError, match="inside fdt_run"):
+            x.fit()
+        with pytest.raises(fdt.FdtError, match="increase"):
+            x.set_grid((4, 4, 4), gidx[::-1].copy())
+        with pytest.raises(fdt.FdtError, match="outside the grid"):
+            x.set_grid((2, 2, 2), gidx)
+        with pytest.raises(fdt.FdtError, match="no neighbour-initialised run"):
+            x.spatial_state()
+    finally:
+        x.close()
+    # --rician and --cnonlinear keep the nonlinear fit (the CPU path fits them nonlinearly too)
+    x = make(fdt, v, nfibres=2, model=1, cpu_init=1, cnonlin=1, njumps=20, nburn=0, sampleevery=10)
+    try:
+        x.fit()
+        P, _ = x.get_params()
+    finally:
+        x.close()
+    assert np.all(np.isfinite(P))
