@@ -80,11 +80,13 @@ def test_tensor_start_gradnonlin(orc, fdt):
         np.testing.assert_allclose(P[i, [0, 1, 2]], e[[0, 1, 2]], rtol=2e-6)
 
 
-def spatial_run(fdt, v, dims, gidx, serial, K=2, **kw):
+def spatial_run(fdt, v, dims, gidx, serial, K=2, grad=None, **kw):
     opts = dict(nfibres=K, model=1, cpu_init=1, njumps=60, nburn=40, sampleevery=6, seed=4242)
     opts.update(kw)
     x = make(fdt, v, spatial_serial=int(serial), **opts)
     try:
+        if grad is not None:
+            x.load_grad_dev(grad)
         x.set_grid(dims, gidx)
         x.run()
         x.sync()
@@ -96,12 +98,16 @@ def spatial_run(fdt, v, dims, gidx, serial, K=2, **kw):
     return out, P, st
 
 
-def test_wavefronts_equal_the_serial_order(fdt):
+@pytest.mark.parametrize("f0,grad", [(False, False), (True, False), (False, True)])
+def test_wavefronts_equal_the_serial_order(fdt, f0, grad):
     dims = (6, 5, 4)
     v, gidx = grid_volume(dims, 0.8, 11)
-    print(f"spatial: {len(gidx)} voxels on {dims}, wavefront vs serial", flush=True)
-    a = spatial_run(fdt, v, dims, gidx, serial=False)
-    b = spatial_run(fdt, v, dims, gidx, serial=True)
+    if grad:  # per-voxel acquisitions (--gradnonlin): the tensor fallback and the chains use each voxel's own
+        v = synth.gradnonlin_voxels("c0", len(gidx))
+    print(f"spatial: {len(gidx)} voxels on {dims}, f0={f0} gradnonlin={grad}, wavefront vs serial", flush=True)
+    kw = dict(f0=int(f0), grad=v["grad"] if grad else None)
+    a = spatial_run(fdt, v, dims, gidx, serial=False, **kw)
+    b = spatial_run(fdt, v, dims, gidx, serial=True, **kw)
     for k in a[0]:
         np.testing.assert_array_equal(a[0][k], b[0][k], err_msg=k)
     np.testing.assert_array_equal(a[1], b[1])
