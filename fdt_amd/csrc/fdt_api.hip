@@ -1253,6 +1253,9 @@ int fdt_enqueue_batch(fdt_queue* q, const fdt_opts* o, int N, const float* bvals
                       int layout, int nvox, uint64_t vox_offset) {
   if (!q || !o) return FDT_EINVAL;
   (void)hipSetDevice(q->device);
+  if (cpu_mode(*o) == 2)
+    return qerr(q, FDT_EUNSUP, "job %zu: neighbour initialisation runs voxel wavefronts one after another and has no "
+                               "place in a batch; use fdt_run, or nospat for the tensor start", q->jobs.size());
   fdt_ctx* c = fdt_create(q->device, o);
   if (!c) return qerr(q, FDT_EINVAL, "job %zu: invalid options or device", q->jobs.size());
   int rc = fdt_set_acquisition(c, N, bvals, bvecs);

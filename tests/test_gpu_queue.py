@@ -141,3 +141,30 @@ def test_gradnonlin_job_in_a_batch(fdt):
         np.testing.assert_array_equal(q.outputs(j)["th"], first["th"])
     finally:
         q.close()
+
+
+def test_tensor_start_job_in_a_batch(fdt):
+    # a CPU-style tensor start (cpu_init + nospat) inside a batch equals its per-volume run;
+    # neighbour initialisation (sequential wavefronts) is refused in a batch
+    q, vols = build_queue(fdt, 30)
+    try:
+        v = synth.voxels("c0", 40)
+        kw = dict(nfibres=2, model=1, nburn=20, njumps=50, sampleevery=25, cpu_init=1, nospat=1)
+        j = q.enqueue(v["bvals"], v["bvecs"], v["data"], vox_offset=7000, **kw)
+        with pytest.raises(fdt.FdtError, match="no place in a batch"):
+            q.enqueue(v["bvals"], v["bvecs"], v["data"], **dict(kw, nospat=0))
+        q.run()
+        got = q.outputs(j)
+        x = fdt.Xfibres(**kw)
+        try:
+            x.set_acquisition(v["bvals"], v["bvecs"])
+            x.load_data(v["data"])
+            x.run(7000)
+            x.sync()
+            want = x.outputs()
+        finally:
+            x.close()
+        for k in ("th", "ph", "f", "mean_S0", "dyads"):
+            np.testing.assert_array_equal(got[k], want[k], err_msg=k)
+    finally:
+        q.close()
