@@ -386,6 +386,33 @@ def bench_spatial(args):
     sp = run(cpu_init=1)
     ts = run(cpu_init=1, nospat=1)
     nl = run(cnonlin=1)
+    conc = None
+    if args.concurrent > 1:
+        # several volumes (further draws of the same distribution on the same grid) at once, one context and
+        # stream each: a neighbour-initialised volume leaves most CUs idle, so volumes share the GPU
+        xs = []
+        try:
+            for j in range(args.concurrent):
+                vj = synth.voxels("c0", nvox, sub=j + 1)
+                x = F.Xfibres(**opts, cpu_init=1)
+                xs.append(x)
+                x.set_acquisition(vj["bvals"], vj["bvecs"])
+                x.load_data(vj["data"])
+                x.set_grid((nx, ny, nz), gidx)
+            for x in xs:
+                x.run(0)
+            for x in xs:
+                x.sync()
+            t = time.perf_counter()
+            for x in xs:
+                x.run(0)
+            for x in xs:
+                x.sync()
+            dt = time.perf_counter() - t
+        finally:
+            for x in xs:
+                x.close()
+        conc = {"volumes": args.concurrent, "voxels_per_s": args.concurrent * nvox / dt, "s": dt}
     line = {
         "metric": "voxels/s, 2-fibre single-shell xfibres with the CPU xfibres neighbour initialisation",
         "value": sp["rate"], "unit": "voxels/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -401,6 +428,8 @@ def bench_spatial(args):
         "dyad_err_median_deg": {"neighbour": sp["dyad"], "tensor": ts["dyad"], "nonlinear": nl["dyad"]},
         "unmatched_fibres": {"neighbour": sp["unmatched"], "tensor": ts["unmatched"], "nonlinear": nl["unmatched"]},
     }
+    if conc:
+        line["concurrent_volumes"] = conc
     print(json.dumps(line), flush=True)
 
 
@@ -559,6 +588,7 @@ def main():
                          "files to the .bedpostX images; spatial: the CPU xfibres neighbour initialisation "
                          "on a slab of the brain-shaped mask")
     ap.add_argument("--slab", type=int, default=16, help="spatial: central z slices of the mask")
+    ap.add_argument("--concurrent", type=int, default=1, help="spatial: also run this many volumes at once")
     ap.add_argument("--oracle-voxels", type=int, default=64)
     ap.add_argument("--train-voxels", type=int, default=2097152)
     ap.add_argument("--epochs", type=int, default=10)

@@ -207,6 +207,7 @@ struct fdt_ctx {
   // neighbour initialisation, in chain fibre order
   int gnx = 0, gny = 0, gnz = 0;
   std::vector<long long> h_gidx;
+  std::vector<int> h_wlist;  // wavefront lists of the last neighbour-initialised run
   DevBuf<long long> d_gidx;
   DevBuf<int> d_vol2mat, d_been, d_wlist;
   DevBuf<float> sp_f, sp_d, sp_S0;
@@ -903,8 +904,12 @@ static int spatial_async(fdt_ctx* c, uint64_t vox_offset) {
       if (start[i + 1] > start[i]) first.push_back(start[i]);
     first.push_back(nv);
   }
-  HIPCHK(c, hipStreamSynchronize(c->stream));  // an earlier run may still read the lists
-  if (nv > 0) HIPCHK(c, hipMemcpy(c->d_wlist.p, order.data(), (size_t)nv * sizeof(int), hipMemcpyHostToDevice));
+  // stream-ordered upload (an earlier run on this stream has finished reading the lists before it lands); the
+  // host copy lives in the context until the next run, which first waits for this one
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->h_wlist.swap(order);
+  if (nv > 0)
+    HIPCHK(c, hipMemcpyAsync(c->d_wlist.p, c->h_wlist.data(), (size_t)nv * sizeof(int), hipMemcpyHostToDevice, c->stream));
   CpuStartArgs st = cpu_start_args(c);
   st.vol2mat = c->d_vol2mat.p;
   st.gidx = c->d_gidx.p;
@@ -1043,8 +1048,10 @@ int fdt_set_grid(fdt_ctx* c, int nx, int ny, int nz, const long long* gidx) {
   for (int v = 0; v < c->nvox; ++v) v2m[(size_t)gidx[v]] = v + 1;
   HIPCHK(c, c->d_vol2mat.ensure((size_t)nxyz));
   HIPCHK(c, c->d_gidx.ensure(std::max(c->nvox, 1)));
-  HIPCHK(c, hipMemcpy(c->d_vol2mat.p, v2m.data(), (size_t)nxyz * sizeof(int), hipMemcpyHostToDevice));
-  if (c->nvox > 0) HIPCHK(c, hipMemcpy(c->d_gidx.p, gidx, (size_t)c->nvox * sizeof(long long), hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpyAsync(c->d_vol2mat.p, v2m.data(), (size_t)nxyz * sizeof(int), hipMemcpyHostToDevice, c->stream));
+  if (c->nvox > 0)
+    HIPCHK(c, hipMemcpyAsync(c->d_gidx.p, gidx, (size_t)c->nvox * sizeof(long long), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
   c->gnx = nx; c->gny = ny; c->gnz = nz;
   c->h_gidx.assign(gidx, gidx + c->nvox);
   return FDT_OK;
