@@ -5,6 +5,7 @@
 #include <sys/stat.h>
 
 #include <atomic>
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <thread>
@@ -122,9 +123,13 @@ bool read_nifti(const std::string& path, Nifti& img, std::string& err) {
   img.sform_code = get<int16_t>(h, H_SFORM, sw);
   for (int i = 0; i < 6; ++i) img.quatern[i] = get<float>(h, H_QUATERN + 4 * i, sw);
   for (int i = 0; i < 12; ++i) img.srow[i] = get<float>(h, H_SROW + 4 * i, sw);
+  // at most 2^38 values (1 TiB as float): larger products come from corrupt headers, and the
+  // check keeps the product from wrapping
+  constexpr size_t kMaxValues = (size_t)1 << 38;
   size_t n = 1;
   for (int i = 1; i <= img.dim[0]; ++i) {
     if (img.dim[i] < 1) { err = path + ": bad dimension"; return false; }
+    if (n > kMaxValues / (size_t)img.dim[i]) { err = path + ": image too large (corrupt dimensions?)"; return false; }
     n *= (size_t)img.dim[i];
   }
   int bytes = 0;
@@ -135,12 +140,24 @@ bool read_nifti(const std::string& path, Nifti& img, std::string& err) {
     case 64: bytes = 8; break;                   // float64
     default: err = path + ": unsupported datatype " + std::to_string(img.datatype); return false;
   }
+  // extensions up to vox_offset are skipped (vox_offset below the header size reads from the header's end)
+  if (!(voxoff < 2147483648.0f)) { err = path + ": bad vox_offset"; return false; }
   size_t skip = voxoff > (float)H_SIZE ? (size_t)voxoff - H_SIZE : 0;
-  std::vector<uint8_t> pad(skip);
-  if (skip && !in.read(pad.data(), skip)) { err = path + ": short file"; return false; }
-  std::vector<uint8_t> raw(n * (size_t)bytes);
+  uint8_t pad[4096];
+  while (skip) {
+    const size_t m = std::min(skip, sizeof pad);
+    if (!in.read(pad, m)) { err = path + ": short file"; return false; }
+    skip -= m;
+  }
+  std::vector<uint8_t> raw;
+  try {
+    raw.resize(n * (size_t)bytes);
+    img.data.resize(n);
+  } catch (const std::bad_alloc&) {
+    err = path + ": not enough host memory for " + std::to_string(n) + " values";
+    return false;
+  }
   if (!in.read(raw.data(), raw.size())) { err = path + ": fewer voxels than the header declares"; return false; }
-  img.data.resize(n);
   switch (img.datatype) {
     case 2: convert<uint8_t>(raw.data(), n, false, img.data.data()); break;
     case 256: convert<int8_t>(raw.data(), n, false, img.data.data()); break;

@@ -122,6 +122,30 @@ def test_read_errors(io, tmp_path):
     assert b"fewer voxels" in err.value
 
 
+def test_corrupt_headers_give_errors_not_aborts(io, tmp_path):
+    # dimensions whose product would wrap, and vox_offset values that are NaN or beyond 2 GiB, are refused
+    # before anything is allocated; a vox_offset below the header size reads from the header's end
+    import struct
+    NI.write(tmp_path / "ok.nii", np.arange(24, dtype=np.float32).reshape(4, 3, 2))
+    raw = bytearray((tmp_path / "ok.nii").read_bytes())
+    err = C.create_string_buffer(256)
+
+    def variant(name, edit):
+        b = bytearray(raw)
+        edit(b)
+        (tmp_path / name).write_bytes(bytes(b))
+        return io.fdt_image_read(str(tmp_path / name).encode(), err, 256)
+
+    assert not variant("dims.nii", lambda b: struct.pack_into("<8h", b, 40, 7, *([32767] * 7)))
+    assert b"too large" in err.value
+    for name, off in (("nan.nii", float("nan")), ("huge.nii", 1e30), ("big.nii", 3e9)):
+        assert not variant(name, lambda b, off=off: struct.pack_into("<f", b, 108, off))
+        assert b"vox_offset" in err.value
+    h = variant("neg.nii", lambda b: struct.pack_into("<f", b, 108, -5.0))
+    assert h
+    io.fdt_image_free(h)
+
+
 # ---------------------------------------------------------------- oracle pins
 def test_dyadic_vectors_pins():
     g = np.random.default_rng(4)
