@@ -184,3 +184,30 @@ def test_errors(fdt):
     finally:
         x.close()
     assert np.all(np.isfinite(P))
+
+
+def test_driver_tensor_start(fdt, tmp_path):
+    """xfibres_gpu --cpuinit --nospat on a bedpostx-style part: the <name>J files hold the library's
+    tensor-started run of the part's global voxel range"""
+    import subprocess
+    nvox, nparts, part = 80, 2, 1
+    v = synth.voxels("c0", nvox)
+    subj = tmp_path / "subj"
+    (tmp_path / "subj.bedpostX" / "logs" / "monitor").mkdir(parents=True)
+    subj.mkdir()
+    np.savetxt(tmp_path / "bvals", v["bvals"][None, :])
+    np.savetxt(tmp_path / "bvecs", v["bvecs"])
+    lo = part * (nvox // nparts)
+    np.ascontiguousarray(v["data"][lo:].T).astype(np.float64).tofile(tmp_path / "data_1")
+    logdir = tmp_path / "parts"
+    cmd = [fdt.CLI_PATH, f"--data={tmp_path / 'data_1'}", f"--mask={tmp_path / 'mask'}", "-b", str(tmp_path / "bvals"),
+           "-r", str(tmp_path / "bvecs"), "--forcedir", f"--logdir={logdir}", "--nf=2", "--bi=50", "--nj=100",
+           "--se=25", "--model=1", "--cpuinit", "--nospat", str(subj), str(part), str(nparts), str(nvox)]
+    print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    th1 = np.fromfile(logdir / "th1samplesJ", dtype=np.float64).reshape(4, nvox - lo)
+    ref = fdt.run_xfibres(v["bvals"], v["bvecs"], v["data"][lo:], nfibres=2, model=1, nburn=50, njumps=100,
+                          sampleevery=25, vox_offset=lo, cpu_init=1, nospat=1)
+    np.testing.assert_array_equal(th1, ref["th"][0].astype(np.float64))
+    assert np.all(ref["tau_init"] == 0)
