@@ -183,7 +183,7 @@ def bench_c4(args):
     """Learned start at high resolution: model 3 (zeppelins) with the Rician
     noise model on a 904,550-voxel brain-shaped volume whose SNR varies
     smoothly in [10, 30] (15% of crossings below 45 deg).  The network is
-    retrained on model-3 labels; the plain path is run as the judge; posterior
+    retrained on model-3 labels; the plain path runs on the same voxels for comparison; posterior
     f is compared with the FP64 oracle on a random sample of voxels."""
     import fdt_amd as F
     from fdt_amd import synth, surrogate as SG
