@@ -33,25 +33,43 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VPL <= 18 ?
 
 }  // namespace fdt
 
-// One translation unit per VPL value (-DFDT_VPL=...), instantiating K=1..3
-// for Gaussian and Rician noise, with the shared acquisition or per-voxel
-// ones (--gradnonlin, a->acqv set; its stride must cover 16 * VPL volumes).
+// Two translation units per VPL value (-DFDT_VPL=...): -DFDT_MCMC_RIC=0 holds
+// the Gaussian kernels and the launcher, -DFDT_MCMC_RIC=1 the Rician ones,
+// so each can be compiled with its own instruction scheduler (the Gaussian
+// kernels gain from an ILP-oriented one, under which the Rician kernels spill
+// several times more than under the default).  K = 1..3, with the shared
+// acquisition or per-voxel ones (--gradnonlin, a->acqv set; its stride must
+// cover 16 * VPL volumes).
 #ifndef FDT_VPL
 #define FDT_VPL 18
 #endif
+#ifndef FDT_MCMC_RIC
+#define FDT_MCMC_RIC 0
+#endif
 #define FDT_CAT2_(a, b) a##b
 #define FDT_CAT2(a, b) FDT_CAT2_(a, b)
+#define FDT_MC_CASE(KK, RR, GG) \
+  if (K == KK && gn == GG) { hipLaunchKernelGGL((fdt::mcmc_kernel<KK, FDT_VPL, RR, GG>), grid, block, 0, s, *a); return hipGetLastError(); }
+
+#if FDT_MCMC_RIC
+extern "C" hipError_t FDT_CAT2(fdt_launch_mcmc_ric_vpl, FDT_VPL)(const fdt::McmcArgs* a, int K, hipStream_t s) {
+  dim3 grid((a->nvox + 15) / 16), block(256);
+  const bool gn = a->acqv != nullptr;
+  FDT_MC_CASE(1, true, false) FDT_MC_CASE(2, true, false) FDT_MC_CASE(3, true, false)
+  FDT_MC_CASE(1, true, true) FDT_MC_CASE(2, true, true) FDT_MC_CASE(3, true, true)
+  return hipErrorInvalidValue;
+}
+#else
+extern "C" hipError_t FDT_CAT2(fdt_launch_mcmc_ric_vpl, FDT_VPL)(const fdt::McmcArgs* a, int K, hipStream_t s);
 extern "C" hipError_t FDT_CAT2(fdt_launch_mcmc_vpl, FDT_VPL)(const fdt::McmcArgs* a, int K, int rician, hipStream_t s) {
   if (a->nvox <= 0) return hipSuccess;
   if (a->acqv && a->acqv_stride < 16 * FDT_VPL) return hipErrorInvalidValue;
+  if (rician) return FDT_CAT2(fdt_launch_mcmc_ric_vpl, FDT_VPL)(a, K, s);
   dim3 grid((a->nvox + 15) / 16), block(256);
   const bool gn = a->acqv != nullptr;
-#define FDT_MC_CASE(KK, RR, GG) \
-  if (K == KK && (rician != 0) == RR && gn == GG) { hipLaunchKernelGGL((fdt::mcmc_kernel<KK, FDT_VPL, RR, GG>), grid, block, 0, s, *a); return hipGetLastError(); }
   FDT_MC_CASE(1, false, false) FDT_MC_CASE(2, false, false) FDT_MC_CASE(3, false, false)
-  FDT_MC_CASE(1, true, false) FDT_MC_CASE(2, true, false) FDT_MC_CASE(3, true, false)
   FDT_MC_CASE(1, false, true) FDT_MC_CASE(2, false, true) FDT_MC_CASE(3, false, true)
-  FDT_MC_CASE(1, true, true) FDT_MC_CASE(2, true, true) FDT_MC_CASE(3, true, true)
-#undef FDT_MC_CASE
   return hipErrorInvalidValue;
 }
+#endif
+#undef FDT_MC_CASE
