@@ -115,17 +115,19 @@ def test_wavefronts_equal_the_serial_order(fdt, f0, grad):
         np.testing.assert_array_equal(a[2][k], b[2][k], err_msg=k)
 
 
-def test_state_and_starts_follow_the_cpu_rules(orc, fdt):
+@pytest.mark.parametrize("K", [2, 3])
+def test_state_and_starts_follow_the_cpu_rules(orc, fdt, K):
+    import itertools
     dims = (7, 6, 5)
     v, gidx = grid_volume(dims, 0.75, 12)
-    nv, K = len(gidx), 2
-    print(f"spatial: {nv} voxels on {dims}", flush=True)
-    out, P, st = spatial_run(fdt, v, dims, gidx, serial=False)
+    nv = len(gidx)
+    print(f"spatial: {nv} voxels on {dims}, nf={K}", flush=True)
+    out, P, st = spatial_run(fdt, v, dims, gidx, serial=False, K=K)
     # per-voxel state against the oracle's finish statistics of the voxel's own samples; the outputs hold the
     # fibres sorted by their sum of f, so the chain order is the permutation whose statistics match
     for i in range(nv):
         ok = False
-        for perm in ([0, 1], [1, 0]):
+        for perm in map(list, itertools.permutations(range(K))):
             b, mf, md, mS0, dy = S.finish_stats(out["th"][perm, :, i], out["ph"][perm, :, i], out["f"][perm, :, i],
                                                 out["d"][:, i], out["S0"][:, i])
             if np.array_equal(mf, st["mean_f"][:, i]):
@@ -145,8 +147,8 @@ def test_state_and_starts_follow_the_cpu_rules(orc, fdt):
     used = src >= 0
     print(f"spatial: {used.mean():.3f} of voxels started from a neighbour", flush=True)
     assert used.mean() > 0.1
-    ang = [3, 4, 6, 7]
-    rest = [0, 1, 2, 5]
+    ang = [3 + 3 * k for k in range(K)] + [4 + 3 * k for k in range(K)]
+    rest = [i for i in range(2 + 3 * K) if i not in ang]
     np.testing.assert_array_equal(P[:, rest], E[:, rest])
     np.testing.assert_allclose(P[:, ang], E[:, ang], atol=1e-6)
     assert np.all(np.isfinite(out["mean_f"]))
