@@ -37,7 +37,7 @@ def make(fdt, v, **opts):
     return x
 
 
-@pytest.mark.parametrize("model,K,f0", [(1, 2, False), (1, 3, True), (2, 2, False)])
+@pytest.mark.parametrize("model,K,f0", [(1, 2, False), (1, 3, True), (2, 2, False), (2, 3, True)])
 def test_tensor_start_matches_oracle(orc, fdt, model, K, f0):
     v = synth.voxels("c0" if model == 1 else "c1", 96)
     x = make(fdt, v, nfibres=K, model=model, f0=int(f0), cpu_init=1, nospat=1, njumps=100, nburn=50, sampleevery=10)
