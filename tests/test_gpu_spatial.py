@@ -213,3 +213,21 @@ def test_driver_tensor_start(fdt, tmp_path):
                           sampleevery=25, vox_offset=lo, cpu_init=1, nospat=1)
     np.testing.assert_array_equal(th1, ref["th"][0].astype(np.float64))
     assert np.all(ref["tau_init"] == 0)
+
+
+@pytest.mark.parametrize("nospat", [0, 1])
+def test_empty_volume(fdt, nospat):
+    v = synth.voxels("c0", 1)
+    x = fdt.Xfibres(nfibres=2, model=1, cpu_init=1, nospat=nospat, njumps=50, nburn=10, sampleevery=10)
+    try:
+        x.set_acquisition(v["bvals"], v["bvecs"])
+        x.load_data(np.zeros((0, v["bvals"].size), np.float32))
+        x.set_grid((3, 3, 3), np.zeros(0, np.int64))
+        x.run()
+        x.sync()
+        out = x.outputs()
+        assert out["f"].shape == (2, 5, 0)
+        if not nospat:
+            assert x.spatial_state()["been"].shape == (0,)
+    finally:
+        x.close()
