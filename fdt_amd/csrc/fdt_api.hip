@@ -331,12 +331,15 @@ int fdt_set_acquisition(fdt_ctx* c, int N, const float* bvals, const float* bvec
   HIPCHK(c, hipMemcpyAsync(c->d_bvecs.p, bvecs, sizeof(float) * 3 * N, hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipMemcpyAsync(c->d_pinv.p, P.data(), sizeof(double) * 7 * N, hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipMemcpyAsync(c->d_acq.p, acq.data(), sizeof(float4) * c->Npad, hipMemcpyHostToDevice, c->stream));
-  // b-shells: distinct b values (exact), used by the MCMC when there are at most 16
+  // b-shells: distinct b values (exact), used by the MCMC when there are at most 16.
+  // Padding slots (i >= N) belong to a b = 0 shell, as their acquisition is
+  // zero: the chain relies on their signals being exactly 1.
   std::vector<float> shb;
   std::vector<int> shell(c->Npad, 0);
-  for (int i = 0; i < N && shb.size() <= 16; ++i) {
-    auto it = std::find(shb.begin(), shb.end(), bvals[i]);
-    if (it == shb.end()) { shb.push_back(bvals[i]); shell[i] = (int)shb.size() - 1; }
+  for (int i = 0; i < c->Npad && shb.size() <= 16; ++i) {
+    const float b = i < N ? bvals[i] : 0.0f;
+    auto it = std::find(shb.begin(), shb.end(), b);
+    if (it == shb.end()) { shb.push_back(b); shell[i] = (int)shb.size() - 1; }
     else shell[i] = (int)(it - shb.begin());
   }
   c->nshell = shb.size() <= 16 ? (int)shb.size() : 0;

@@ -35,6 +35,25 @@ FDT_DEV float ex2(float x) { return __builtin_amdgcn_exp2f(x); }
 FDT_DEV float lg2(float x) { return __builtin_amdgcn_logf(x); }
 FDT_DEV float flog(float x) { return lg2(x) * LN2; }
 
+// sin and cos of a float angle of any magnitude.  Width adaptation drives the
+// angles of unsupported fibres to 1e8..1e11 rad, and the reference evaluates
+// them as sin/cos(double(angle)) with a full-range reduction
+// (CUDA/runmcmc_kernels.cu:340, :1165, :1177-1182).  Here the angle is turned
+// into revolutions x/(2 pi) and reduced to its fraction in double, with
+// 1/(2 pi) split into two doubles: the first FMA forms x * HI exactly before
+// subtracting the nearest integer, so the fraction is accurate to ~2^-53 for
+// |x| < 2^60.  The hardware sin/cos take revolutions (valid in [-256, 256]).
+FDT_DEV void sincos_full(float x, float& s, float& c) {
+  constexpr double R_HI = 0x1.45f306dc9c883p-3, R_LO = -0x1.6b01ec5417056p-57;
+  const double xd = (double)x;
+  const double n = __builtin_rint(xd * R_HI);
+  double fr = __builtin_fma(xd, R_HI, -n);
+  fr = __builtin_fma(xd, R_LO, fr);
+  const float f = (float)fr;
+  s = __builtin_amdgcn_sinf(f);
+  c = __builtin_amdgcn_cosf(f);
+}
+
 // draw slots (fixed per parameter)
 enum { SL_F0 = 0, SL_TAU = 1, SL_D = 2, SL_DSTD = 3, SL_R = 4, SL_S0 = 5, SL_FIB = 6 };
 
@@ -148,8 +167,11 @@ struct Chain {
   int nshell;
   float sval, sprop, sprior;
   int scnt;
+  // sin and cos of the slot's angle (theta_k / phi_k slots), kept with the
+  // value so that fibre directions are row broadcasts, not re-evaluated
+  float ssin, scos;
   double like;  // Gaussian: SSR; Rician: sum of ell
-  bool last_ok;  // the lane's last slot holds a real volume
+  int nv;       // slots of this lane that hold a real volume (rl + 16 t < N)
   int rl;
   // options
   int model;
@@ -195,15 +217,22 @@ struct Chain {
     Dirs D;
     sfor<K>([&](auto kc) {
       constexpr int k = kc;
-      float st, ct, sp, cp;
-      __sincosf(v<TH(k)>(), &st, &ct);
-      __sincosf(v<PH(k)>(), &sp, &cp);
+      const float st = row_bcast<TH(k)>(ssin), ct = row_bcast<TH(k)>(scos);
+      const float sp = row_bcast<PH(k)>(ssin), cp = row_bcast<PH(k)>(scos);
       D.x[k][0] = st * cp; D.x[k][1] = st * sp; D.x[k][2] = ct;
       xform(D.x[k][0], D.x[k][1], D.x[k][2]);
     });
     return D;
   }
-  FDT_DEV float mask_last(int t, float x) const { return (t == VPL - 1 && !last_ok) ? 0.0f : x; }
+  // Contribution of slot t: zero where the slot is padding (volume >= N).
+  // Padding slots see g = 0 and b = 0 in every acquisition table (shared,
+  // per-voxel, b-shell), so their isotropic and stick signals are exactly 1
+  // for every parameter value.  The Gaussian chain keeps their residual at
+  // exactly 0 (masked where it is formed and where S0 or the mixture moves),
+  // so the f0, f and angle proposals, whose per-volume change is then exactly
+  // 0 there, need no mask; the S0 and diffusivity proposals, where the
+  // mixture at a padding slot moves by rounding, and every Rician term do.
+  FDT_DEV float mask(int t, float x) const { return t < nv ? x : 0.0f; }
   // the fractions of the row: fk[k] and f0, their sum
   struct Fr {
     float fk[K];
@@ -227,8 +256,9 @@ struct Chain {
       return halfN * lp;
     }
   }
-  FDT_DEV static float theta_prior(float t) {  // fibre.h:283-290
-    return t == 0.0f ? 0.0f : -flog(fabsf(__sinf(t)) * 0.5f);
+  // theta prior -log(|sin t| / 2) from st = sin t (fibre.h:283-290)
+  FDT_DEV static float theta_prior(float t, float st) {
+    return t == 0.0f ? 0.0f : -flog(fabsf(st) * 0.5f);
   }
   // R prior (fibre.h:816-843) at diffusivity d; false outside its support
   FDT_DEV bool r_prior(float Rv, float d, float& pr) const {
@@ -263,10 +293,10 @@ struct Chain {
       M[t] = m;
       if constexpr (RIC) {
         ell[t] = ell_of(r[t], S0 * m, tau);
-        ss += (double)mask_last(t, ell[t]);
+        ss += (double)mask(t, ell[t]);
       } else {
-        r[t] = y[t] - S0 * m;
-        ss += (double)mask_last(t, r[t] * r[t]);
+        r[t] = mask(t, y[t] - S0 * m);
+        ss += (double)(r[t] * r[t]);
       }
     }
     like = (double)row_sum16((float)ss);
@@ -289,8 +319,8 @@ struct Chain {
     for (int t = 0; t < VPL; ++t) {
       if constexpr (RIC) {
         en[t] = ell_of(r[t], S0 * fmaf(df, 1.0f - iso[t], M[t]), tau);
-        a += mask_last(t, en[t] - ell[t]);
-      } else qd.add(r[t], mask_last(t, 1.0f - iso[t]));
+        a += mask(t, en[t] - ell[t]);
+      } else qd.add(r[t], 1.0f - iso[t]);
     }
     a = RIC ? row_sum16(a) : qd.delta(S0 * df);
     if (dlik(a) + (prn - prior<SL_F0>()) < -lu) {
@@ -313,7 +343,7 @@ struct Chain {
       const float S0 = v<SL_S0>();
       float a = 0.0f, en[VPL];
 #pragma unroll
-      for (int t = 0; t < VPL; ++t) { en[t] = ell_of(r[t], S0 * M[t], tn); a += mask_last(t, en[t] - ell[t]); }
+      for (int t = 0; t < VPL; ++t) { en[t] = ell_of(r[t], S0 * M[t], tn); a += mask(t, en[t] - ell[t]); }
       a = row_sum16(a);
       float dE = -Nf * (flog(tn) - flog(tau)) - a;
       if (dE < -lu) {
@@ -334,14 +364,14 @@ struct Chain {
     Quad qd;
 #pragma unroll
     for (int t = 0; t < VPL; ++t) {
-      if constexpr (RIC) { en[t] = ell_of(r[t], Sn * M[t], tau); a += mask_last(t, en[t] - ell[t]); }
-      else qd.add(r[t], mask_last(t, M[t]));
+      if constexpr (RIC) { en[t] = ell_of(r[t], Sn * M[t], tau); a += mask(t, en[t] - ell[t]); }
+      else qd.add(r[t], mask(t, M[t]));
     }
     a = RIC ? row_sum16(a) : qd.delta(dS);
     if (dlik(a) < -lu) {
 #pragma unroll
       for (int t = 0; t < VPL; ++t) {
-        if constexpr (RIC) ell[t] = en[t]; else r[t] -= dS * M[t];
+        if constexpr (RIC) ell[t] = en[t]; else r[t] -= dS * mask(t, M[t]);
       }
       like += (double)a;
       accept<SL_S0>(Sn, 0.0f);
@@ -459,8 +489,8 @@ struct Chain {
         for (int t = 0; t < VPL; ++t) {
           float mn = mix_new<decltype(SA)::value, decltype(SI)::value, decltype(NA)::value, decltype(NI)::value,
                              decltype(SH)::value>(t, q, ca, ci, D);
-          if constexpr (RIC) a += mask_last(t, ell_of(r[t], S0 * mn, tau) - ell[t]);
-          else qd.add(r[t], mask_last(t, mn - M[t]));
+          if constexpr (RIC) a += mask(t, ell_of(r[t], S0 * mn, tau) - ell[t]);
+          else qd.add(r[t], mask(t, mn - M[t]));
         }
       });
     });
@@ -476,7 +506,7 @@ struct Chain {
             store_new<decltype(SA)::value, decltype(SI)::value, decltype(NA)::value, decltype(NI)::value,
                       decltype(SH)::value>(t, ca, ci);
             float mn = mix(t, q);
-            if constexpr (RIC) ell[t] = ell_of(r[t], S0 * mn, tau); else r[t] -= S0 * (mn - M[t]);
+            if constexpr (RIC) ell[t] = ell_of(r[t], S0 * mn, tau); else r[t] -= S0 * mask(t, mn - M[t]);
             M[t] = mn;
           }
         });
@@ -489,12 +519,15 @@ struct Chain {
   template <int k, bool IS_TH>
   FDT_DEV void prop_angle(float n, float lu) {
     constexpr int SL = IS_TH ? TH(k) : PH(k);
-    float tn = v<TH(k)>(), pn = v<PH(k)>(), prn = 0.0f, prold = 0.0f, xn;
-    if constexpr (IS_TH) { tn = xn = tn + n * width<SL>(); prn = theta_prior(tn); prold = prior<SL>(); }
-    else pn = xn = pn + n * width<SL>();
-    float st, ct, sp, cp;
-    __sincosf(tn, &st, &ct);
-    __sincosf(pn, &sp, &cp);
+    constexpr int SO = IS_TH ? PH(k) : TH(k);  // the other angle's slot: its sin/cos are cached there
+    const float xn = v<SL>() + n * width<SL>();
+    float sn_, cn_;
+    sincos_full(xn, sn_, cn_);
+    const float so = row_bcast<SO>(ssin), co = row_bcast<SO>(scos);
+    float prn = 0.0f, prold = 0.0f;
+    if constexpr (IS_TH) { prn = theta_prior(xn, sn_); prold = prior<SL>(); }
+    const float st = IS_TH ? sn_ : so, ct = IS_TH ? cn_ : co;
+    const float sp = IS_TH ? so : sn_, cp = IS_TH ? co : cn_;
     float x0 = st * cp, x1 = st * sp, x2 = ct;
     xform(x0, x1, x2);
     const SigC ca = aniso_coef(model, v<SL_D>(), v<SL_DSTD>(), v<SL_R>());
@@ -515,8 +548,8 @@ struct Chain {
     for (int t = 0; t < VPL; ++t) {
       if constexpr (RIC) {
         en[t] = ell_of(r[t], S0 * fmaf(fk, sn[t] - s[k][t], M[t]), tau);
-        a += mask_last(t, en[t] - ell[t]);
-      } else qd.add(r[t], mask_last(t, sn[t] - s[k][t]));
+        a += mask(t, en[t] - ell[t]);
+      } else qd.add(r[t], sn[t] - s[k][t]);
     }
     a = RIC ? row_sum16(a) : qd.delta(S0 * fk);
     if (dlik(a) + (prn - prold) < -lu) {
@@ -529,6 +562,7 @@ struct Chain {
         s[k][t] = sn[t];
       }
       like += (double)a;
+      if (rl == SL) { ssin = sn_; scos = cn_; }
       if constexpr (IS_TH) accept<SL>(xn, prn);
       else if (rl == SL) { sval = xn; scnt += 1; }
     } else reject<SL>();
@@ -549,8 +583,8 @@ struct Chain {
     for (int t = 0; t < VPL; ++t) {
       if constexpr (RIC) {
         en[t] = ell_of(r[t], S0 * fmaf(df, s[k][t] - iso[t], M[t]), tau);
-        a += mask_last(t, en[t] - ell[t]);
-      } else qd.add(r[t], mask_last(t, s[k][t] - iso[t]));
+        a += mask(t, en[t] - ell[t]);
+      } else qd.add(r[t], s[k][t] - iso[t]);
     }
     a = RIC ? row_sum16(a) : qd.delta(S0 * df);
     if (dlik(a) + (prn - prior<SL>()) < -lu) {
@@ -639,7 +673,7 @@ FDT_DEV void run_chain(const McmcArgs& a, int vox, const ChainLds& L) {
   c.halfN = 0.5f * (float)a.N;
   c.Nf = (float)a.N;
   c.Rmean = a.Rmean; c.Rstd = a.Rstd; c.Rfudge = a.Rfudge;
-  c.last_ok = (rl + 16 * (VPL - 1)) < a.N;
+  c.nv = a.N > rl ? min(VPL, (a.N - rl + 15) >> 4) : 0;
 #pragma unroll
   for (int k = 0; k < K; ++k) c.lam[k] = (k == 0) ? (a.all_ard != 0) : (a.no_ard == 0);
 
@@ -679,6 +713,7 @@ FDT_DEV void run_chain(const McmcArgs& a, int vox, const ChainLds& L) {
     if (a.model == 3) Rr = a.Rmean;
   }
   float val = 0.0f, pr = 0.0f, wd = 0.2f;  // f0 and the fibre slots start at width 0.2 (:497-503)
+  float vs = 0.0f, vc = 1.0f;              // sin / cos of an angle slot's value
   if (rl == SL_F0) {
     val = c.incf0 ? P[a.np - 1] : 0.0f;
     if (c.incf0 && c.ardf0 && val > 0.0f && val < 1.0f) pr = flog(val);
@@ -704,8 +739,8 @@ FDT_DEV void run_chain(const McmcArgs& a, int vox, const ChainLds& L) {
     wd = val / 10.0f;
   } else if (rl < SL_FIB + 3 * K) {
     int k = (rl - SL_FIB) / 3, w = (rl - SL_FIB) % 3;
-    if (w == 0) { val = P[3 + 3 * k + add]; pr = C::theta_prior(val); }
-    else if (w == 1) val = P[4 + 3 * k + add];
+    if (w == 0) { val = P[3 + 3 * k + add]; sincos_full(val, vs, vc); pr = C::theta_prior(val, vs); }
+    else if (w == 1) { val = P[4 + 3 * k + add]; sincos_full(val, vs, vc); }
     else {
       val = P[2 + 3 * k + add];
       bool lk = (k == 0) ? (a.all_ard != 0) : (a.no_ard == 0);
@@ -719,6 +754,8 @@ FDT_DEV void run_chain(const McmcArgs& a, int vox, const ChainLds& L) {
   c.sval = val;
   c.sprior = pr;
   c.sprop = wd;
+  c.ssin = vs;
+  c.scos = vc;
   c.scnt = 0;
   {
     SigC ca = aniso_coef(a.model, d, dstd, Rr), ci = iso_coef(a.model, d, dstd);
@@ -803,7 +840,7 @@ FDT_DEV void run_chain(const McmcArgs& a, int vox, const ChainLds& L) {
 #pragma unroll
       for (int t = 0; t < VPL; ++t) {
         const float e = c.r[t] - S0 * c.M[t];
-        acc += c.mask_last(t, e * e);
+        acc += c.mask(t, e * e);
       }
       ss = (double)row_sum16(acc);
     } else {
