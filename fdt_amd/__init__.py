@@ -30,7 +30,7 @@ EXPORTS = ["fdt_default_opts", "fdt_create", "fdt_destroy", "fdt_last_error", "f
            "fdt_surrogate_create", "fdt_surrogate_destroy", "fdt_surrogate_last_error", "fdt_surrogate_num_weights",
            "fdt_surrogate_load", "fdt_surrogate_save", "fdt_surrogate_set_training_data", "fdt_surrogate_gradients",
            "fdt_surrogate_train_step", "fdt_surrogate_forward", "fdt_surrogate_infer", "fdt_run_surrogate", "fdt_get_fallback_flags",
-           "fdt_predict", "fdt_set_grid", "fdt_get_spatial_state"]
+           "fdt_predict", "fdt_set_grid", "fdt_get_spatial_state", "fdt_get_lm_iters"]
 
 
 class Opts(C.Structure):
@@ -91,6 +91,7 @@ def lib():
         L.fdt_lm_stage.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float, _FP, _FP, ip]
         L.fdt_lm_start.argtypes = [vp, C.c_int, C.c_int, C.c_int, _FP]
         L.fdt_get_params.argtypes = [vp, _FP, _FP]
+        L.fdt_get_lm_iters.argtypes = [vp, ip]
         L.fdt_set_params.argtypes = [vp, _FP, _FP]
         L.fdt_mcmc.argtypes = [vp, C.c_uint64]
         L.fdt_finish.argtypes = [vp]
@@ -247,6 +248,12 @@ class Xfibres:
         t = np.zeros(self.nvox, np.float32)
         self._chk(self.L.fdt_get_params(self.h, p.ctypes.data_as(_FP), t.ctypes.data_as(_FP)))
         return p, t
+
+    def lm_iters(self):
+        """LM loop trips per voxel of the last fit (all stages)"""
+        it = np.zeros(self.nvox, np.int32)
+        self._chk(self.L.fdt_get_lm_iters(self.h, it.ctypes.data_as(C.POINTER(C.c_int))))
+        return it
 
     def set_params(self, params, tau=None):
         p, pp = _f32(params)

@@ -32,6 +32,9 @@ FDT_LM_DECL(3, 0)
 FDT_LM_DECL(1, 1)
 FDT_LM_DECL(2, 1)
 FDT_LM_DECL(3, 1)
+FDT_LM_DECL(1, 2)
+FDT_LM_DECL(2, 2)
+FDT_LM_DECL(3, 2)
 hipError_t fdt_launch_f0_flag(const F0FlagArgs*, hipStream_t);
 hipError_t fdt_launch_zero_ints(int*, int, hipStream_t);
 hipError_t fdt_launch_mcmc_queue(const QueueArgs*, int rows_per_wg, int max_items, hipStream_t);
@@ -187,6 +190,7 @@ struct fdt_ctx {
   bool use_widths_in = false;
   DevBuf<unsigned char> d_refit;  // voxels refitted without f0
   bool have_params = false, have_samples = false, have_finish = false;
+  bool have_iters = false;  // d_iters holds the LM iterations of the current parameters
   // raw and sorted samples
   DevBuf<float> s_th, s_ph, s_f, s_S0, s_d, s_dstd, s_R, s_f0, s_tau;
   DevBuf<float> o_th, o_ph, o_f;
@@ -240,6 +244,7 @@ static int check_opts(fdt_ctx* c, const fdt_opts& o) {
   if (o.sampleevery <= 0) return set_err(c, FDT_EINVAL, "sampleevery must be > 0");
   if (o.updateproposalevery <= 0) return set_err(c, FDT_EINVAL, "updateproposalevery must be > 0");
   if (o.njumps > 0 && nsamples_of(o) == 0) return set_err(c, FDT_EINVAL, "njumps < sampleevery gives no samples");
+  if (o.lm_fp64 < 0 || o.lm_fp64 > 2) return set_err(c, FDT_EINVAL, "lm_fp64=%d: must be 0, 1 or 2", o.lm_fp64);
   return FDT_OK;
 }
 
@@ -256,6 +261,7 @@ void fdt_default_opts(fdt_opts* o) {
   o->nburn = 1000;
   o->sampleevery = 25;
   o->updateproposalevery = 40;
+  o->lm_fp64 = 2;  // FP32 normal equations, FP64 parameters and cost
   o->seed = 8665904;
   o->R_prior_mean = 0.13f;
   o->R_prior_std = 0.03f;
@@ -433,28 +439,37 @@ int fdt_get_voxel_acquisition(fdt_ctx* c, int vox, float* bvals, float* bvecs) {
   return FDT_OK;
 }
 
-static hipError_t launch_lm(const LmArgs& a, int K, int fp64, int kind, int f0, hipStream_t s) {
-  if (!fp64) {
+// prec: 0 FP32, 1 FP64, 2 FP32 normal equations with FP64 parameters and cost
+static hipError_t launch_lm(const LmArgs& a, int K, int prec, int kind, int f0, hipStream_t s) {
+  if (prec == 0) {
     if (K == 1) return fdt_launch_lm_k1_0(&a, kind, f0, s);
     if (K == 2) return fdt_launch_lm_k2_0(&a, kind, f0, s);
     if (K == 3) return fdt_launch_lm_k3_0(&a, kind, f0, s);
-  } else {
+  } else if (prec == 1) {
     if (K == 1) return fdt_launch_lm_k1_1(&a, kind, f0, s);
     if (K == 2) return fdt_launch_lm_k2_1(&a, kind, f0, s);
     if (K == 3) return fdt_launch_lm_k3_1(&a, kind, f0, s);
+  } else if (prec == 2) {
+    if (K == 1) return fdt_launch_lm_k1_2(&a, kind, f0, s);
+    if (K == 2) return fdt_launch_lm_k2_2(&a, kind, f0, s);
+    if (K == 3) return fdt_launch_lm_k3_2(&a, kind, f0, s);
   }
   return hipErrorInvalidValue;
 }
 
-static hipError_t launch_tau(const TauArgs& a, int K, int fp64, hipStream_t s) {
-  if (!fp64) {
+static hipError_t launch_tau(const TauArgs& a, int K, int prec, hipStream_t s) {
+  if (prec == 0) {
     if (K == 1) return fdt_launch_tau_k1_0(&a, s);
     if (K == 2) return fdt_launch_tau_k2_0(&a, s);
     if (K == 3) return fdt_launch_tau_k3_0(&a, s);
-  } else {
+  } else if (prec == 1) {
     if (K == 1) return fdt_launch_tau_k1_1(&a, s);
     if (K == 2) return fdt_launch_tau_k2_1(&a, s);
     if (K == 3) return fdt_launch_tau_k3_1(&a, s);
+  } else if (prec == 2) {
+    if (K == 1) return fdt_launch_tau_k1_2(&a, s);
+    if (K == 2) return fdt_launch_tau_k2_2(&a, s);
+    if (K == 3) return fdt_launch_tau_k3_2(&a, s);
   }
   return hipErrorInvalidValue;
 }
@@ -539,7 +554,7 @@ static int tensor_start_async(fdt_ctx* c) {
   HIPCHK(c, fdt_launch_cpu_start(&a, c->stream));
   if (!c->quiet) HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
   c->timed_fit = !c->quiet;
-  c->have_params = true;
+  c->have_params = true; c->have_iters = false;
   c->have_samples = c->have_finish = false;
   return FDT_OK;
 }
@@ -558,7 +573,11 @@ static int fit_async(fdt_ctx* c) {
                                     "the neighbours before it); set nospat for the tensor start");
     return tensor_start_async(c);
   }
-  const int K = o.nfibres, fp64 = o.lm_fp64 ? 1 : 0, f0 = o.f0 ? 1 : 0;
+  // --f0 fits run the FP64 normal equations in the mixed mode: the f0 column
+  // is nearly collinear with the isotropic ones and the FP32 Hessian steers
+  // them off the FP64 fits (within 1e-4 of the FP64 oracle on 45% of voxels
+  // against 87% on the multi-shell volume)
+  const int K = o.nfibres, f0 = o.f0 ? 1 : 0, fp64 = (o.lm_fp64 == 2 && f0) ? 1 : o.lm_fp64;
   const int np = np_of(o), np1 = 2 + 3 * K + f0;
   size_t nv = std::max(c->nvox, 1);
   HIPCHK(c, c->d_params.ensure(nv * np));
@@ -654,7 +673,7 @@ static int fit_async(fdt_ctx* c) {
   }
   if (!c->quiet) HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
   c->timed_fit = !c->quiet;
-  c->have_params = true;
+  c->have_params = true; c->have_iters = true;
   c->have_samples = c->have_finish = false;
   return FDT_OK;
 }
@@ -694,7 +713,7 @@ int fdt_lm_stage(fdt_ctx* c, int kind, int K, int f0, int G, float R, const floa
   a.out = dout.p;
   a.out_np = np;
   a.iters = dit.p;
-  HIPCHK(c, launch_lm(a, K, c->o.lm_fp64 ? 1 : 0, kind, f0 ? 1 : 0, c->stream));
+  HIPCHK(c, launch_lm(a, K, c->o.lm_fp64, kind, f0 ? 1 : 0, c->stream));
   HIPCHK(c, hipMemcpyAsync(out, dout.p, n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
   if (iters) HIPCHK(c, hipMemcpyAsync(iters, dit.p, c->nvox * sizeof(int), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -720,7 +739,7 @@ int fdt_lm_start(fdt_ctx* c, int kind, int K, int f0, float* out) {
   a.queue = c->d_queue.p;
   a.out = dout.p;
   a.out_np = np;
-  HIPCHK(c, launch_lm(a, K, c->o.lm_fp64 ? 1 : 0, kind, f0 ? 1 : 0, c->stream));
+  HIPCHK(c, launch_lm(a, K, c->o.lm_fp64, kind, f0 ? 1 : 0, c->stream));
   HIPCHK(c, hipMemcpyAsync(out, dout.p, n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   dout.release();
@@ -734,6 +753,16 @@ int fdt_get_params(fdt_ctx* c, float* params, float* tau) {
   size_t n = (size_t)c->nvox * np_of(c->o);
   if (params && n) HIPCHK(c, hipMemcpyAsync(params, c->d_params.p, n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
   if (tau && c->nvox) HIPCHK(c, hipMemcpyAsync(tau, c->d_tau.p, c->nvox * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return FDT_OK;
+}
+
+int fdt_get_lm_iters(fdt_ctx* c, int* iters) {
+  if (!c || !iters) return set_err(c, FDT_EINVAL, "null argument");
+  if (!c->have_params || !c->have_iters)
+    return set_err(c, FDT_ESTATE, "no LM iteration counts: the current parameters do not come from fdt_fit");
+  (void)hipSetDevice(c->device);
+  if (c->nvox) HIPCHK(c, hipMemcpyAsync(iters, c->d_iters.p, c->nvox * sizeof(int), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return FDT_OK;
 }
@@ -752,7 +781,7 @@ int fdt_set_params(fdt_ctx* c, const float* params, const float* tau) {
     else HIPCHK(c, hipMemsetAsync(c->d_tau.p, 0, c->nvox * sizeof(float), c->stream));
   }
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  c->have_params = true;
+  c->have_params = true; c->have_iters = false;
   c->have_samples = c->have_finish = false;
   return FDT_OK;
 }
@@ -882,7 +911,7 @@ static int spatial_async(fdt_ctx* c, uint64_t vox_offset) {
   HIPCHK(c, c->sp_dyad.ensure(nvs * K * 3));
   HIPCHK(c, fdt_launch_zero_ints(c->d_been.p, nv, c->stream));
   HIPCHK(c, fdt_launch_zero_ints(reinterpret_cast<int*>(c->d_tau.p), nv, c->stream));
-  c->have_params = true;
+  c->have_params = true; c->have_iters = false;
   McmcArgs a;
   mcmc_launch_fn fn = nullptr;
   if (int rc = mcmc_prepare(c, vox_offset, &a, &fn)) return rc;
@@ -1774,7 +1803,7 @@ int fdt_surrogate_infer(fdt_surrogate* s, fdt_ctx* c) {
   if (int rc = sg_infer(s, c, dir)) return rc;
   SHIP(s, hipStreamSynchronize(c->stream));
   dir.release();
-  c->have_params = true;
+  c->have_params = true; c->have_iters = false;
   c->use_widths_in = true;
   c->have_samples = c->have_finish = false;
   return FDT_OK;
@@ -1801,7 +1830,7 @@ int fdt_run_surrogate(fdt_ctx* c, fdt_surrogate* s, uint64_t vox_offset, int nbu
   if (int rc = sg_infer(s, c, dir)) return set_err(c, rc, "%s", s->err.c_str());
   HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
   c->timed_fit = true;
-  c->have_params = true;
+  c->have_params = true; c->have_iters = false;
   // short chains from the learned start
   c->use_widths_in = true;
   McmcArgs a;
@@ -1857,12 +1886,12 @@ int fdt_run_surrogate(fdt_ctx* c, fdt_surrogate* s, uint64_t vox_offset, int nbu
     if (o.model == 1) {
       l.out = c->d_params.p;
       l.out_np = np;
-      HIPCHK(c, launch_lm(l, K, o.lm_fp64 ? 1 : 0, k1, f0, c->stream));
+      HIPCHK(c, launch_lm(l, K, o.lm_fp64, k1, f0, c->stream));
     } else {
       HIPCHK(c, c->d_s1.ensure((size_t)nv * np1));
       l.out = c->d_s1.p;
       l.out_np = np1;
-      HIPCHK(c, launch_lm(l, K, o.lm_fp64 ? 1 : 0, k1, f0, c->stream));
+      HIPCHK(c, launch_lm(l, K, o.lm_fp64, k1, f0, c->stream));
       LmArgs m = l;
       m.init = INIT_STAGE1;
       m.in = c->d_s1.p;
@@ -1871,7 +1900,7 @@ int fdt_run_surrogate(fdt_ctx* c, fdt_surrogate* s, uint64_t vox_offset, int nbu
       m.queue = q + 1;
       m.out = c->d_params.p;
       m.out_np = np;
-      HIPCHK(c, launch_lm(m, K, o.lm_fp64 ? 1 : 0, KIND_MULTI, f0, c->stream));
+      HIPCHK(c, launch_lm(m, K, o.lm_fp64, KIND_MULTI, f0, c->stream));
     }
     if (o.rician) {
       // tau from the LM fits (recomputed for every voxel; only the rerun ones read it again)
@@ -1886,7 +1915,7 @@ int fdt_run_surrogate(fdt_ctx* c, fdt_surrogate* s, uint64_t vox_offset, int nbu
       t.f0 = f0;
       t.R = o.R_prior_mean;
       t.tau = c->d_tau.p;
-      HIPCHK(c, launch_tau(t, K, o.lm_fp64 ? 1 : 0, c->stream));
+      HIPCHK(c, launch_tau(t, K, o.lm_fp64, c->stream));
     }
     // their chains with the configured burn-in and default widths
     McmcArgs b = a;

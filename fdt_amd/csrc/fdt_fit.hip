@@ -436,41 +436,53 @@ FDT_DEV void row_normal_eq(const Row& R, const Mdl<T, KIND, K, F0>& m, const T* 
 // per-row state: lambda0 = 0.1, /10 after a success, x10 after a failure (the
 // Hessian diagonal carries lambda - lambda_old), relative cost test 1e-8,
 // ltol 1e20, stop at the 201st success test.
-template <class T, int KIND, int K, bool F0>
+// T is the type of the Jacobian rows and the normal equations, TC that of the
+// parameters and of the cost.  TC = double with T = float (the mixed mode)
+// keeps every decision of the loop -- accept/reject and the 1e-8 relative
+// stop test -- on an FP64 cost (an FP32 model's cost carries ~1e-7 relative
+// noise, above the stop tolerance) while the O(N np^2) normal equations stay
+// in FP32: they set the step direction only.  (Residuals of the FP64 model in
+// the gradient, measured, changed nothing: the FP32 Jacobian sets where the
+// steps lead along the flat directions of the fit.)
+template <class T, class TC, int KIND, int K, bool F0>
 struct LmState {
   static constexpr int NP = Mdl<T, KIND, K, F0>::NP;
-  T p[NP];
+  TC p[NP];
   double Hrow[NP], grow;
   double pcf, lambda, olambda;
   int niter, total;
   bool success;
 
-  FDT_DEV void begin(const Row& R, const Mdl<T, KIND, K, F0>& m0) {
-    Mdl<T, KIND, K, F0> m = m0;
+  FDT_DEV void begin(const Row& R, const Mdl<TC, KIND, K, F0>& mc0) {
+    Mdl<TC, KIND, K, F0> m = mc0;
     m.prep(p);
-    pcf = row_cost<T>(R, m, p);
+    pcf = row_cost<TC>(R, m, p);
     lambda = 0.1; olambda = 0.0;
     niter = 0; total = 0;
     success = true;
   }
   // one trip of the loop; returns true when the stage has ended
-  FDT_DEV bool step(const Row& R, const Mdl<T, KIND, K, F0>& m0) {
+  FDT_DEV bool step(const Row& R, const Mdl<T, KIND, K, F0>& m0, const Mdl<TC, KIND, K, F0>& mc0) {
     if (success && niter++ >= 200) return true;
     ++total;
-    Mdl<T, KIND, K, F0> m = m0;
     if (success) {
-      m.prep(p);
-      row_normal_eq<T, KIND, K, F0>(R, m, p, Hrow, grow);
+      Mdl<T, KIND, K, F0> m = m0;
+      T pj[NP];
+#pragma unroll
+      for (int i = 0; i < NP; ++i) pj[i] = (T)p[i];
+      m.prep(pj);
+      row_normal_eq<T, KIND, K, F0>(R, m, pj, Hrow, grow);
     }
 #pragma unroll
     for (int i = 0; i < NP; ++i) if (R.rl == i) Hrow[i] += (lambda - olambda);
     double sol[NP];
     lu_row<NP>(Hrow, grow, sol, R.rl);
-    T st[NP];
+    TC st[NP];
 #pragma unroll
-    for (int i = 0; i < NP; ++i) st[i] = p[i] - (T)sol[i];
-    m.prep(st);
-    double ncf = row_cost<T>(R, m, st);
+    for (int i = 0; i < NP; ++i) st[i] = p[i] - (TC)sol[i];
+    Mdl<TC, KIND, K, F0> mc = mc0;
+    mc.prep(st);
+    double ncf = row_cost<TC>(R, mc, st);
     bool end = false;
     if ((success = (ncf < pcf))) {
       olambda = 0.0;
@@ -968,8 +980,8 @@ FDT_DEV void store_vec(float* dst, const T (&v)[NP], int rl) {
 
 // GN: --gradnonlin, every voxel reads its own corrected acquisition from
 // a.acqv instead of the workgroup's table in LDS.
-template <class T, int KIND, int K, bool F0, bool GN>
-__global__ __launch_bounds__(256) void lm_kernel(LmArgs a) {
+template <class T, class TC, int KIND, int K, bool F0, bool GN>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(std::is_same<T, TC>::value && std::is_same<T, double>::value ? 1 : 2, 2))) void lm_kernel(LmArgs a) {
   extern __shared__ float4 lds4[];
   float4* sacq = lds4;
   float* sy = reinterpret_cast<float*>(lds4 + a.N);
@@ -986,8 +998,12 @@ __global__ __launch_bounds__(256) void lm_kernel(LmArgs a) {
   m0.G = a.G;
   m0.Rv = (T)a.R;
   m0.invR = T(1) / (T(2) * (T)a.R + T(1));
+  Mdl<TC, KIND, K, F0> mc0;
+  mc0.G = a.G;
+  mc0.Rv = (TC)a.R;
+  mc0.invR = TC(1) / (TC(2) * (TC)a.R + TC(1));
   const int count = a.count ? *a.count : a.nvox;
-  LmState<T, KIND, K, F0> S;
+  LmState<T, TC, KIND, K, F0> S;
   int vox = 0;
   bool fresh = true;
   for (;;) {
@@ -1015,28 +1031,28 @@ __global__ __launch_bounds__(256) void lm_kernel(LmArgs a) {
       }
       if (a.init == INIT_BUF) {
 #pragma unroll
-        for (int i = 0; i < NP; ++i) S.p[i] = (T)a.in[(size_t)vox * a.in_np + i];
+        for (int i = 0; i < NP; ++i) S.p[i] = (TC)a.in[(size_t)vox * a.in_np + i];
       } else if constexpr (KIND == KIND_MULTI) {
-        T s1[NP - 1];
+        TC s1[NP - 1];
 #pragma unroll
-        for (int i = 0; i < NP - 1; ++i) s1[i] = (T)a.in[(size_t)vox * a.in_np + i];
-        start_multi<T, K, F0>(s1, S.p);
+        for (int i = 0; i < NP - 1; ++i) s1[i] = (TC)a.in[(size_t)vox * a.in_np + i];
+        start_multi<TC, K, F0>(s1, S.p);
       } else {
         Dti dti;
         if constexpr (GN) dti = dti_row_gn(R);
         else dti = dti_row(R, a.pinvA);
-        if constexpr (KIND == KIND_SINGLE) start_single<T, K, F0>(dti, S.p);
-        else start_single_c<T, K, F0>(dti, R, S.p);
+        if constexpr (KIND == KIND_SINGLE) start_single<TC, K, F0>(dti, S.p);
+        else start_single_c<TC, K, F0>(dti, R, S.p);
       }
       if (!a.lm) {
         store_vec<NP>(a.out + (size_t)vox * a.out_np, S.p, rl);
         continue;
       }
-      S.begin(R, m0);
+      S.begin(R, mc0);
       fresh = false;
     }
-    if (S.step(R, m0)) {
-      finalise<T, KIND, K, F0>(S.p);
+    if (S.step(R, m0, mc0)) {
+      finalise<TC, KIND, K, F0>(S.p);
       store_vec<NP>(a.out + (size_t)vox * a.out_np, S.p, rl);
       if (a.set_f0_slot && rl == 15) a.out[(size_t)vox * a.out_np + a.out_np - 1] = 0.001f;
       if (a.iters && rl == 0) a.iters[vox] = (a.iters_add ? a.iters[vox] : 0) + S.total;
@@ -1197,7 +1213,8 @@ __global__ __launch_bounds__(256) void cpu_start_kernel(CpuStartArgs a) {
 }  // namespace fdt
 
 // One translation unit per (K, precision), compiled in parallel:
-// -DFDT_FIT_K=1..3 -DFDT_FIT_FP64=0/1.
+// -DFDT_FIT_K=1..3 -DFDT_FIT_FP64=0 (FP32), 1 (FP64) or 2 (FP32 normal
+// equations with FP64 parameters and cost).
 #ifndef FDT_FIT_K
 #define FDT_FIT_K 3
 #endif
@@ -1206,13 +1223,14 @@ __global__ __launch_bounds__(256) void cpu_start_kernel(CpuStartArgs a) {
 #endif
 #define FDT_CAT3_(a, b, c) a##b##_##c
 #define FDT_CAT3(a, b, c) FDT_CAT3_(a, b, c)
-using FitReal = std::conditional<FDT_FIT_FP64 != 0, double, float>::type;
+using FitReal = std::conditional<FDT_FIT_FP64 == 1, double, float>::type;
+using FitCost = std::conditional<FDT_FIT_FP64 != 0, double, float>::type;
 
 namespace {
 // persistent grid: as many workgroups as stay resident, at most one row per voxel
 template <int KIND, bool F0, bool GN>
 hipError_t launch_lm(const fdt::LmArgs* a, hipStream_t s) {
-  auto kern = fdt::lm_kernel<FitReal, KIND, FDT_FIT_K, F0, GN>;
+  auto kern = fdt::lm_kernel<FitReal, FitCost, KIND, FDT_FIT_K, F0, GN>;
   size_t lds = (size_t)a->N * sizeof(float4) + (size_t)fdt::LM_ROWS * (16 * (((a->N + 15) / 16) | 1)) * sizeof(float);
   int dev = 0, cus = 0, per_cu = 0;
   hipError_t e;
@@ -1248,7 +1266,7 @@ extern "C" hipError_t FDT_CAT3(fdt_launch_tau_k, FDT_FIT_K, FDT_FIT_FP64)(const 
   if (a->nvox <= 0) return hipSuccess;
   if (a->N > fdt::MAX_N) return hipErrorInvalidValue;
   dim3 grid((a->nvox + 15) / 16), block(256);
-  hipLaunchKernelGGL((fdt::tau_kernel<FitReal, FDT_FIT_K>), grid, block, 0, s, *a);
+  hipLaunchKernelGGL((fdt::tau_kernel<FitCost, FDT_FIT_K>), grid, block, 0, s, *a);
   return hipGetLastError();
 }
 

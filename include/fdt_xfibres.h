@@ -66,7 +66,10 @@ typedef struct fdt_opts {
   float R_prior_mean;      /* --Rmean */
   float R_prior_std;       /* --Rstd */
   float R_prior_fudge;     /* --Rfudge */
-  int lm_fp64;             /* 0: FP32 model/Jacobian (production); 1: FP64 throughout (parity check) */
+  int lm_fp64;             /* LM arithmetic: 0 FP32 model, Jacobian and cost; 1 FP64 throughout;
+                              2 (default) FP32 Jacobian and normal equations with FP64 parameters and cost
+                              (every accept/reject and stop decision on an FP64 cost); with --f0, 2 runs
+                              as 1 */
   int cpu_init;            /* 1: start the chains as the CPU xfibres does (xfibres.cc:517-541): without
                               nonlinear fitting for Gaussian noise when neither nonlin nor cnonlin is set,
                               from the tensor (nospat) or from the best finished neighbour (needs the
@@ -171,6 +174,14 @@ int fdt_lm_start(fdt_ctx* c, int kind, int K, int f0, float* out);
  * CUDA/xfibres_gpu.cu:88-128); it is exposed so a caller can start the chains
  * from its own initialisation. */
 int fdt_get_params(fdt_ctx* c, float* params, float* tau);
+
+/* LM loop trips per voxel (successful and failed iterations, summed over the
+ * stages and the --f0 refit) of the last fdt_fit / fdt_run: iters [nvox].  The
+ * reference counts them inside levenberg_marquardt_PVM_*_gpu
+ * (CUDA/levenberg_marquardt.cu:24-333) without reporting them; they are
+ * exposed for the run report (iteration histogram).  FDT_ESTATE when the
+ * current parameters were not produced by the LM fit. */
+int fdt_get_lm_iters(fdt_ctx* c, int* iters);
 int fdt_set_params(fdt_ctx* c, const float* params, const float* tau);
 
 /* Burn-in + record as one persistent launch (CUDA/runmcmc.cu:108-316).

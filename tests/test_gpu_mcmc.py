@@ -5,7 +5,7 @@ counters, so from identical LM parameters the chains agree draw for draw
 until a near-tie accept/reject decision flips; the statistical bar of the
 hot path (posterior-mean dyads within 2 degrees on 95% of voxels with 2+
 supported fibres, |mean f| within 0.02, same supported-fibre counts on 98%)
-is checked on full-length chains."""
+is measured on full-length chains beside the oracle's own floors."""
 import os
 import subprocess
 
@@ -13,6 +13,7 @@ import numpy as np
 import pytest
 
 from fdt_amd import synth
+import posterior as PS
 
 pytestmark = pytest.mark.gpu
 
@@ -32,10 +33,10 @@ def run_engine(fdt, v, data, params, tau, vox_offset=0, **opts):
         x.close()
 
 
-def oracle_chain(orc, v, data, params, tau, **opts):
+def oracle_chain(orc, v, data, params, tau, prec=64, **opts):
     o = orc.opts(**opts)
     s = orc.mcmc(o, v["bvals"], v["bvecs"], data.astype(np.float64), params.astype(np.float32).astype(np.float64),
-                 None if tau is None else tau.astype(np.float32).astype(np.float64))
+                 None if tau is None else tau.astype(np.float32).astype(np.float64), prec=prec)
     fin = orc.finish(o, s)
     fin.update(s)
     return fin
@@ -61,35 +62,37 @@ def test_short_chain_tracks_oracle_draw_for_draw(orc, fdt, model, K):
     assert same.mean() >= 0.75
 
 
-def supported(mean_f):
-    return (mean_f > 0.05).sum(0)
-
-
-@pytest.mark.parametrize("model,K,nvox", [(1, 2, 192), (2, 3, 96)])
-def test_posterior_parity_with_oracle(orc, fdt, model, K, nvox):
-    name = "c0" if model == 1 else "c1"
+@pytest.mark.parametrize("name,model,K,nvox", [("c0", 1, 2, 1024), ("c1", 2, 3, 768)])
+def test_posterior_parity_with_oracle(orc, fdt, name, model, K, nvox):
+    """Full-length chains (1,000 burn-in + 1,250 recorded sweeps) on the GPU
+    against the FP64 oracle.  The north-star bar (ARD counts equal on 98% of
+    voxels, |mean f| within 0.02 and dyads within 2 degrees on 95%) is not met
+    by the oracle against itself: with another seed (Monte Carlo noise of 50
+    recorded samples), or in the reference's FP32 arithmetic with the same
+    draws.  The GPU chain is therefore held to those floors: with the same
+    seed it tracks the FP64 oracle at least as well as the FP32 restatement
+    does, and against the oracle under another seed it agrees as well as the
+    oracle agrees with itself; where a floor meets the bar, the bar itself is
+    asserted."""
     v = synth.voxels(name, nvox)
     base = dict(nfibres=K, model=model)
     params, tau, _ = orc.fit(orc.opts(**base), v["bvals"], v["bvecs"], v["data"].astype(np.float64))
     run = dict(base, nburn=1000, njumps=1250, sampleevery=25)
+    seed_b = 12345
     g = run_engine(fdt, v, v["data"], params, tau, **run)
     o = oracle_chain(orc, v, v["data"], params, tau, **run)
-    ng, no = supported(g["mean_f"]), supported(o["mean_f"])
-    ard_agree = (ng == no).mean()
-    df = np.abs(g["mean_f"] - o["mean_f"])
-    two = (ng >= 2) & (no >= 2)
-    ang = []
-    for i in np.nonzero(two)[0]:
-        for k in range(2):
-            c = abs(g["dyads"][k, :, i] @ o["dyads"][k, :, i])
-            ang.append(np.degrees(np.arccos(min(1.0, c))))
-    ang = np.array(ang)
-    print(f"model {model}: ARD agreement {ard_agree:.3f}, |df| p95 {np.percentile(df, 95):.4f}, "
-          f"dyad angle (2+ fibres, n={two.sum()}) p95 {np.percentile(ang, 95) if ang.size else 0:.2f} deg")
-    assert ard_agree >= 0.9
-    assert (df <= 0.02).mean() >= 0.9
-    if ang.size:
-        assert (ang <= 2.0).mean() >= 0.85
+    o32 = oracle_chain(orc, v, v["data"], params, tau, prec=32, **run)
+    ob = oracle_chain(orc, v, v["data"], params, tau, seed=seed_b, **run)
+    track, prec, floor, cross = PS.stats(g, o), PS.stats(o32, o), PS.stats(ob, o), PS.stats(g, ob)
+    for lab, s in (("GPU vs oracle, same seed", track), ("oracle FP32 vs FP64, same seed", prec),
+                   ("oracle vs oracle, other seed", floor), ("GPU vs oracle, other seed", cross)):
+        print(f"{name} model {model} K={K}, {nvox} voxels, {lab}: {PS.fmt(s)}")
+    for m in ("ard", "df", "dyad"):
+        n = track["n2"] if m == "dyad" else track["n"]
+        assert track[m] >= prec[m] - PS.tol(prec[m], n), (m, track[m], prec[m])
+        assert cross[m] >= floor[m] - PS.tol(floor[m], n), (m, cross[m], floor[m])
+        if prec[m] >= PS.BAR[m]:
+            assert track[m] >= PS.BAR[m] - PS.tol(PS.BAR[m], n), (m, track[m])
     np.testing.assert_allclose(g["mean_S0"], o["mean_S0"], rtol=0.02)
 
 

@@ -54,4 +54,9 @@ def test_bench_helpers():
     import bench
     # the flop model of the default workload: 82,944 per voxel and sweep (model 2, K=3, N=288)
     assert bench.flops_per_jump(2, 3, 288) == 82944
-    assert bench.SG_DEFAULTS["nburn_short"] == 50 and bench.SG_DEFAULTS_C4["nburn_short"] == 200
+    # the learned start as the learned-start workload specifies it (burn-in 200), with the burn-in-50 variant
+    assert bench.SG_DEFAULTS["nburn_short"] == 200 and bench.SG_SHORT["nburn_short"] == 50
+    assert bench.SG_DEFAULTS_C4["nburn_short"] == 200
+    # LM flop model at the multi-shell size: 288 * (12*13 + 24 + 120) per loop trip
+    assert bench.lm_flops_per_iter(288, 12) == 288 * 300
+    assert bench.LM_DEFAULT in bench.LM_MODES
