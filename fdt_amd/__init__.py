@@ -30,7 +30,7 @@ EXPORTS = ["fdt_default_opts", "fdt_create", "fdt_destroy", "fdt_last_error", "f
            "fdt_surrogate_create", "fdt_surrogate_destroy", "fdt_surrogate_last_error", "fdt_surrogate_num_weights",
            "fdt_surrogate_load", "fdt_surrogate_save", "fdt_surrogate_set_training_data", "fdt_surrogate_gradients",
            "fdt_surrogate_train_step", "fdt_surrogate_forward", "fdt_surrogate_infer", "fdt_run_surrogate", "fdt_get_fallback_flags",
-           "fdt_predict", "fdt_set_grid", "fdt_get_spatial_state", "fdt_get_lm_iters"]
+           "fdt_predict", "fdt_set_grid", "fdt_get_spatial_state", "fdt_get_lm_iters", "fdt_mcmc_noalloc"]
 
 
 class Opts(C.Structure):
@@ -92,6 +92,8 @@ def lib():
         L.fdt_lm_start.argtypes = [vp, C.c_int, C.c_int, C.c_int, _FP]
         L.fdt_get_params.argtypes = [vp, _FP, _FP]
         L.fdt_get_lm_iters.argtypes = [vp, ip]
+        L.fdt_mcmc_noalloc.argtypes = [C.POINTER(Opts), C.c_int, vp, vp, vp, vp, vp, C.c_uint64, C.c_int,
+                                       C.POINTER(Outputs), vp, C.c_char_p, C.c_size_t]
         L.fdt_set_params.argtypes = [vp, _FP, _FP]
         L.fdt_mcmc.argtypes = [vp, C.c_uint64]
         L.fdt_finish.argtypes = [vp]

@@ -30,6 +30,7 @@
 #ifndef FDT_XFIBRES_H
 #define FDT_XFIBRES_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -315,6 +316,30 @@ int fdt_get_fallback_flags(fdt_ctx* c, unsigned char* flags);
 int fdt_predict(int device, int N, const float* bvals, const float* bvecs, int nvox, int K, int model, const float* S0,
                 const float* d, const float* dstd, const float* R, const float* f0, const float* f, const float* dyads,
                 float* out);
+
+/* Allocation-free chains and finishing.  No counterpart in the reference:
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Runs what
+ * fdt_mcmc + fdt_finish run, for nvox voxels held in the caller's device
+ * memory, without a context: d_bvals [N] and d_bvecs [3][N] (as
+ * fdt_set_acquisition takes them), d_data [nvox][N] voxel-major, d_params
+ * [nvox][np] (the layout of fdt_get_params), d_tau [nvox] (--rician, else
+ * NULL).  Every pointer of *out is a device pointer sized as for
+ * fdt_get_outputs; th/ph/f, the means, dyads, disp and the scalar samples of
+ * the model's parameters are required (the scalar samples feed the means).
+ * The library's scratch -- the acquisition table and the unsorted fibre
+ * samples -- is a static device array on each GPU (held until the process
+ * ends): 3 * nfibres * (njumps / sampleevery) * nvox must not exceed
+ * FDT_NOALLOC_SAMPLE_FLOATS, else FDT_EINVAL.  Calls no hipMalloc/hipFree;
+ * every launch is ordered on `stream` (a hipStream_t; NULL is the null
+ * stream) and nothing synchronises the host.  Calls that overlap on one GPU
+ * share the scratch array and race.  On error the message is written to msg
+ * (msg_len bytes, may be NULL).  Outputs equal those of fdt_set_params +
+ * fdt_mcmc + fdt_finish + fdt_get_outputs on the same inputs. */
+#define FDT_NOALLOC_SAMPLE_FLOATS (4u << 20) /* 16 MiB of unsorted fibre samples */
+int fdt_mcmc_noalloc(const fdt_opts* o, int N, const float* d_bvals, const float* d_bvecs, const float* d_data,
+                     const float* d_params, const float* d_tau, uint64_t vox_offset, int nvox,
+                     const fdt_outputs* out, void* stream, char* msg, size_t msg_len);
 
 #ifdef __cplusplus
 }
