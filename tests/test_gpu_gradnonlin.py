@@ -87,10 +87,10 @@ def run_chains(fdt, v, grad, params, tau, vox_offset=0, **opts):
         x.close()
 
 
-def oracle_chains(orc, v, grad, params, tau, **opts):
+def oracle_chains(orc, v, grad, params, tau, prec=64, **opts):
     o = orc.opts(**opts)
     s = orc.mcmc(o, f32(v["bvals"]), f32(v["bvecs"]), v["data"].astype(np.float64), f32(params),
-                 None if tau is None else f32(tau), grad=grad)
+                 None if tau is None else f32(tau), grad=grad, prec=prec)
     fin = orc.finish(o, s)
     fin.update(s)
     return fin
@@ -113,19 +113,26 @@ def test_short_chain_tracks_oracle_draw_for_draw(orc, fdt, model, K, rician):
 
 
 def test_posterior_parity_with_oracle(orc, fdt):
-    v = synth.gradnonlin_voxels("c1", 96, sd=0.04)
+    """full chains with per-voxel corrected acquisitions against the FP64
+    oracle, held to the oracle's own floors (its FP32 restatement with the
+    same draws; itself with another seed), as the shared-acquisition chains"""
+    import posterior as PS
+    v = synth.gradnonlin_voxels("c1", 256, sd=0.04)
     base = dict(nfibres=3, model=2)
     params, tau, _ = orc.fit(orc.opts(**base), f32(v["bvals"]), f32(v["bvecs"]), v["data"].astype(np.float64),
                              grad=v["grad"])
     run = dict(base, nburn=1000, njumps=1250, sampleevery=25)
     g = run_chains(fdt, v, v["grad"], params, None, **run)
     o = oracle_chains(orc, v, v["grad"], params, None, **run)
-    ng, no = (g["mean_f"] > 0.05).sum(0), (o["mean_f"] > 0.05).sum(0)
-    df = np.abs(g["mean_f"] - o["mean_f"])
-    print(f"ARD agreement {(ng == no).mean():.3f}, |df| p95 {np.percentile(df, 95):.4f}")
-    assert (ng == no).mean() >= 0.9
-    assert (df <= 0.02).mean() >= 0.9
-    np.testing.assert_allclose(g["mean_S0"], o["mean_S0"], rtol=0.02)
+    o32 = oracle_chains(orc, v, v["grad"], params, None, prec=32, **run)
+    ob = oracle_chains(orc, v, v["grad"], params, None, seed=12345, **run)
+    track, prec, floor, cross = PS.stats(g, o), PS.stats(o32, o), PS.stats(ob, o), PS.stats(g, ob)
+    print(f"--gradnonlin: GPU vs oracle {PS.fmt(track)} | FP32 vs FP64 {PS.fmt(prec)} | other seed: "
+          f"oracle {PS.fmt(floor)}, GPU {PS.fmt(cross)}")
+    for m in ("ard", "df", "dyad"):
+        n = max(track["n2"], 1) if m == "dyad" else track["n"]
+        assert track[m] >= prec[m] - PS.tol(prec[m], n), (m, track[m], prec[m])
+        assert cross[m] >= floor[m] - PS.tol(floor[m], n), (m, cross[m], floor[m])
 
 
 def test_errors_and_reset(fdt):

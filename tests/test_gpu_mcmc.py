@@ -93,7 +93,10 @@ def test_posterior_parity_with_oracle(orc, fdt, name, model, K, nvox):
         assert cross[m] >= floor[m] - PS.tol(floor[m], n), (m, cross[m], floor[m])
         if prec[m] >= PS.BAR[m]:
             assert track[m] >= PS.BAR[m] - PS.tol(PS.BAR[m], n), (m, track[m])
-    np.testing.assert_allclose(g["mean_S0"], o["mean_S0"], rtol=0.02)
+    s0 = np.abs(g["mean_S0"] / o["mean_S0"] - 1)
+    s0f = np.abs(ob["mean_S0"] / o["mean_S0"] - 1)
+    print(f"mean S0 within 2% of the oracle: GPU {np.mean(s0 <= 0.02):.3f}, oracle other seed {np.mean(s0f <= 0.02):.3f}")
+    assert np.mean(s0 <= 0.02) >= np.mean(s0f <= 0.02) - PS.tol(np.mean(s0f <= 0.02), len(s0))
 
 
 def test_seed_determinism_and_partition_independence(orc, fdt):

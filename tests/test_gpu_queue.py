@@ -7,6 +7,7 @@ import numpy as np
 import pytest
 
 from fdt_amd import synth
+import posterior as PS
 
 pytestmark = pytest.mark.gpu
 
@@ -59,6 +60,8 @@ def test_graph_replay_repeats_first_run(fdt):
 
 
 def test_autotune_then_run(fdt):
+    """after the autotuner has chosen workgroup sizes and tiles, every job equals
+    its per-volume run at the smallest tile (padding slots contribute nothing)"""
     q, vols = build_queue(fdt, 48)
     try:
         ms = q.autotune(jumps=20, sample_vox=48)
@@ -68,11 +71,11 @@ def test_autotune_then_run(fdt):
         for j, (name, model, K) in enumerate(MIX):
             assert 16 * t["vpl"][j] >= vols[j][0]["bvals"].size
         q.run()
-        for j, (name, model, K) in enumerate(MIX):
-            out = q.outputs(j)
-            f = out["f"]
-            assert np.all(np.isfinite(f)) and np.all((f > 0) & (f < 1))
-            np.testing.assert_allclose(np.linalg.norm(out["dyads"], axis=1), 1.0, rtol=1e-5)
+        for j, (v, kw) in enumerate(vols):
+            got = q.outputs(j)
+            want = fdt.run_xfibres(v["bvals"], v["bvecs"], v["data"], vox_offset=1000 * j, **kw)
+            for k in ("th", "ph", "f", "mean_S0", "mean_d", "dyads"):
+                np.testing.assert_array_equal(got[k], want[k], err_msg=f"job {j} {k}")
     finally:
         q.close()
 
@@ -166,5 +169,103 @@ def test_tensor_start_job_in_a_batch(fdt):
             x.close()
         for k in ("th", "ph", "f", "mean_S0", "dyads"):
             np.testing.assert_array_equal(got[k], want[k], err_msg=k)
+    finally:
+        q.close()
+
+
+def oracle_chain(orc, v, params, tau, vox_offset, prec=64, **run):
+    o = orc.opts(**run)
+    smp = orc.mcmc(o, v["bvals"], v["bvecs"], v["data"].astype(np.float64), params.astype(np.float64),
+                   None if tau is None else tau.astype(np.float64), vox_offset=vox_offset, prec=prec)
+    fin = orc.finish(o, smp)
+    fin.update(smp)
+    return fin
+
+
+def test_queue_job_types_track_oracle_draw_for_draw(orc, fdt):
+    """every job type of the mixed workload (N 33/69/145/288, models 1/2/2/3,
+    nf 1/2/3/3) in one batch: from the queue's own LM start, the first recorded
+    sample matches the FP64 oracle's chain draw for draw on most voxels"""
+    run = dict(nburn=0, njumps=50, sampleevery=25)
+    q = fdt.Queue()
+    vols = []
+    try:
+        for j, (name, model, K) in enumerate(MIX):
+            v = synth.voxels(name, 64, sub=7)
+            q.enqueue(v["bvals"], v["bvecs"], v["data"], vox_offset=1000 * j, nfibres=K, model=model, **run)
+            vols.append((v, model, K))
+        q.run()
+        for j, (v, model, K) in enumerate(vols):
+            g = q.outputs(j)
+            params, _ = q.params(j)
+            o = oracle_chain(orc, v, params, None, 1000 * j, nfibres=K, model=model, **run)
+            same = np.all(np.abs(g["f"][:, 0, :] - o["f"][:, 0, :]) < 1e-3, axis=0) & \
+                (np.abs(g["d"][0] / o["d"][0] - 1) < 1e-3)
+            print(f"job {j} ({MIX[j]}): first sample agrees on {same.mean():.3f} of voxels")
+            assert same.mean() >= 0.75
+    finally:
+        q.close()
+
+
+def test_queue_job_types_posterior_vs_oracle(orc, fdt):
+    """full chains (1,000 + 1,250 sweeps) of every job type in one batch
+    against the FP64 oracle from the same start, held to the oracle's own
+    floors: the FP32 restatement with the same draws, and the oracle with
+    another seed"""
+    run = dict(nburn=1000, njumps=1250, sampleevery=25)
+    q = fdt.Queue()
+    vols = []
+    try:
+        for j, (name, model, K) in enumerate(MIX):
+            v = synth.voxels(name, 96, sub=8)
+            q.enqueue(v["bvals"], v["bvecs"], v["data"], vox_offset=1000 * j, nfibres=K, model=model, **run)
+            vols.append((v, model, K))
+        q.run()
+        for j, (v, model, K) in enumerate(vols):
+            g = q.outputs(j)
+            params, _ = q.params(j)
+            kw = dict(nfibres=K, model=model, **run)
+            o = oracle_chain(orc, v, params, None, 1000 * j, **kw)
+            o32 = oracle_chain(orc, v, params, None, 1000 * j, prec=32, **kw)
+            ob = oracle_chain(orc, v, params, None, 1000 * j, seed=12345, **kw)
+            track, prec, floor, cross = PS.stats(g, o), PS.stats(o32, o), PS.stats(ob, o), PS.stats(g, ob)
+            print(f"job {j} {MIX[j]}: GPU vs oracle {PS.fmt(track)} | FP32 vs FP64 {PS.fmt(prec)} | "
+                  f"other seed: oracle {PS.fmt(floor)}, GPU {PS.fmt(cross)}")
+            for m in ("ard", "df", "dyad"):
+                n = max(track["n2"], 1) if m == "dyad" else track["n"]
+                assert track[m] >= prec[m] - PS.tol(prec[m], n), (j, m, track[m], prec[m])
+                assert cross[m] >= floor[m] - PS.tol(floor[m], n), (j, m, cross[m], floor[m])
+            s0 = np.abs(g["mean_S0"] / o["mean_S0"] - 1)
+            s0f = np.abs(ob["mean_S0"] / o["mean_S0"] - 1)
+            assert np.mean(s0 <= 0.02) >= np.mean(s0f <= 0.02) - PS.tol(np.mean(s0f <= 0.02), len(s0)), (j, s0.max())
+    finally:
+        q.close()
+
+
+def test_full_size_mixed_queue(fdt):
+    """the mixed workload at full size: four volumes of 60,000 voxels in one
+    batch with the default chains; every output within its bounds"""
+    q = fdt.Queue()
+    try:
+        for j, (name, model, K) in enumerate(MIX):
+            v = synth.voxels(name, 60000, with_truth=False)
+            q.enqueue(v["bvals"], v["bvecs"], v["data"], vox_offset=100000 * j, nfibres=K, model=model)
+        del v
+        q.run()
+        for j, (name, model, K) in enumerate(MIX):
+            out = q.outputs(j)
+            f = out["f"]
+            assert f.shape == (K, 50, 60000)
+            assert np.all(np.isfinite(f)) and np.all((f > 0) & (f < 1)), j
+            assert np.all(f.sum(0) <= 1 + 1e-5), j
+            np.testing.assert_allclose(np.linalg.norm(out["dyads"], axis=1), 1.0, rtol=1e-5)
+            assert np.all((out["d"] >= 0) & (out["d"] <= 0.005)) and np.all(out["S0"] >= 0), j
+            assert np.all(np.isfinite(out["th"])) and np.all(np.isfinite(out["ph"])), j
+            if model >= 2:
+                up = 0.01 if model == 2 else 0.004
+                assert np.all((out["dstd"] > 0) & (out["dstd"] <= up)), j
+            if model == 3:
+                assert np.all((out["R"] > 0) & (out["R"] <= 1)), j
+            print(f"job {j} {MIX[j]}: mean f1 {out['mean_f'][0].mean():.3f}, mean S0 {out['mean_S0'].mean():.1f}")
     finally:
         q.close()

@@ -110,43 +110,106 @@ def test_surrogate_run_untrained_falls_back(fdt):
 
 
 def test_learned_start_gains_over_plain_at_short_burnin(fdt):
-    """Trained on the engine's own labels (65,536 voxels), evaluated on other
-    voxels: median dyad error vs the truth with burn-in 200 from the learned
-    start must not be worse than the plain LM start with the same burn-in, and
-    within 0.5 deg of the plain path with the full burn-in of 1000."""
+    """What the trained network adds over the method without it: the committed
+    multi-shell weights (trained by this engine's own MFMA kernels on 2,097,152
+    labelled voxels, bench.py --workload c3) against the plain LM start with the
+    same short burn-in, on 16,384 voxels disjoint from the training set.  At
+    burn-in 200 and 50 the learned start's median dyad error vs the truth must
+    be lower than the plain start's (a gain, not a tolerated loss), with no more
+    fibres left unmatched, and within 0.5 deg of the plain path's full burn-in."""
+    import os
     from fdt_amd import surrogate as SG
+    ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    wz = np.load(os.path.join(ROOT, "fdt_amd", "weights", "surrogate_c1_model2_nf3.npz"))
     run = dict(nfibres=3, model=2, njumps=1250, sampleevery=25)
-    tr = synth.voxels("c1", 65536, sub=1, with_truth=False)
-    t = SG.label(tr["bvals"], tr["bvecs"], tr["data"], nburn=1000, **run)
-    s = fdt.Surrogate(tr["bvals"].size, seed=11)
-    ev = synth.voxels("c1", 4096, sub=2)
+    ev = synth.voxels("c1", 16384, sub=2)
+    s = fdt.Surrogate(ev["bvals"].size, seed=0)
     try:
-        losses = SG.train(s, tr["data"], tr["bvals"], t, epochs=8, batch=512, lr=1e-3, lr_decay=0.75)
-        print("losses", [f"{x:.3f}" for x in losses])
+        s.load(wz["weights"])
 
         def errs(out):
             e = SG.dyad_errors(out["dyads"], out["mean_f"], ev["dirs"], ev["f"])
             return np.nanmedian(e), np.mean(np.isnan(e))
 
-        plain_short = fdt.run_xfibres(ev["bvals"], ev["bvecs"], ev["data"], nburn=200, **run)
         plain_full = fdt.run_xfibres(ev["bvals"], ev["bvecs"], ev["data"], nburn=1000, **run)
-        x = fdt.Xfibres(nburn=1000, **run)
+        mf, nf = errs(plain_full)
+        for bi in (200, 50):
+            plain_short = fdt.run_xfibres(ev["bvals"], ev["bvecs"], ev["data"], nburn=bi, **run)
+            x = fdt.Xfibres(nburn=1000, **run)
+            try:
+                x.set_acquisition(ev["bvals"], ev["bvecs"])
+                x.load_data(ev["data"])
+                nfb = x.run_surrogate(s, nburn_short=bi, chi2_max=3.0, ang_max_deg=30.0)
+                learned = x.outputs()
+            finally:
+                x.close()
+            ms, ns_ = errs(plain_short)
+            ml, nl = errs(learned)
+            print(f"burn-in {bi}: median dyad error plain {ms:.3f} deg (unmatched {ns_:.4f}), learned {ml:.3f} "
+                  f"(unmatched {nl:.4f}, fallback {nfb}/16384); plain bi=1000 {mf:.3f} (unmatched {nf:.4f})")
+            assert ml < ms
+            assert nl <= ns_ + 0.01
+            assert ml <= mf + 0.5
+    finally:
+        s.close()
+
+
+def test_learned_path_model3_rician_vs_oracle(orc, fdt):
+    """The high-resolution model-3 + Rician workload: voxels of its generator
+    (SNR field, close crossings) through the plain path and through the
+    learned path (committed model-3 weights, burn-in 200, fallback), both
+    against the FP64 oracle's posterior (its own LM fit and chains) on 128
+    voxels.  Each GPU path differs from the oracle in its start and draws, so
+    it is held to the oracle-vs-oracle floor with another seed; the fallback
+    rate is bounded."""
+    import os
+    import posterior as PS
+    ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    wz = np.load(os.path.join(ROOT, "fdt_amd", "weights", "surrogate_c4_model3_rician_nf3.npz"))
+    full = synth.voxels("c4", 4096)
+    idx = np.sort(np.random.default_rng(41).choice(4096, 128, replace=False))
+    v = {k: (full[k][idx] if isinstance(full[k], np.ndarray) and full[k].shape[:1] == (4096,) else full[k])
+         for k in full}
+    run = dict(nfibres=3, model=3, rician=1, nburn=1000, njumps=1250, sampleevery=25)
+    plain = fdt.run_xfibres(v["bvals"], v["bvecs"], v["data"], **run)
+    s = fdt.Surrogate(v["bvals"].size, seed=0)
+    try:
+        s.load(wz["weights"])
+        x = fdt.Xfibres(**run)
         try:
-            x.set_acquisition(ev["bvals"], ev["bvecs"])
-            x.load_data(ev["data"])
-            nfb = x.run_surrogate(s, nburn_short=200)
+            x.set_acquisition(v["bvals"], v["bvecs"])
+            x.load_data(v["data"])
+            nfb = x.run_surrogate(s, nburn_short=200, chi2_max=3.0, ang_max_deg=30.0)
             learned = x.outputs()
         finally:
             x.close()
-        ms, ns_ = errs(plain_short)
-        mf, nf = errs(plain_full)
-        ml, nl = errs(learned)
-        print(f"median dyad error: plain bi=200 {ms:.2f} deg (missed {ns_:.3f}), plain bi=1000 {mf:.2f} "
-              f"(missed {nf:.3f}), learned start bi=200 {ml:.2f} (missed {nl:.3f}); fallback {nfb}/4096")
-        assert ml <= ms + 0.1
-        assert ml <= mf + 0.5
     finally:
         s.close()
+    y = v["data"].astype(np.float64)
+
+    def oracle(seed):
+        o = orc.opts(seed=seed, **run)
+        p, tau, _ = orc.fit(o, v["bvals"], v["bvecs"], y)
+        smp = orc.mcmc(o, v["bvals"], v["bvecs"], y, p.astype(np.float32).astype(np.float64),
+                       tau.astype(np.float32).astype(np.float64))
+        fin = orc.finish(o, smp)
+        fin.update(smp)
+        return fin
+
+    oa, ob = oracle(8665904), oracle(12345)
+    floor, pl, le = PS.stats(ob, oa), PS.stats(plain, oa), PS.stats(learned, oa)
+    print(f"model 3 + Rician, 128 voxels: oracle other seed {PS.fmt(floor)}; plain GPU {PS.fmt(pl)}; "
+          f"learned GPU {PS.fmt(le)}; fallback {nfb}/128")
+    for m in ("ard", "df", "dyad"):
+        n = max(floor["n2"], 1) if m == "dyad" else floor["n"]
+        assert pl[m] >= floor[m] - PS.tol(floor[m], n), ("plain", m, pl[m], floor[m])
+        assert le[m] >= floor[m] - PS.tol(floor[m], n), ("learned", m, le[m], floor[m])
+    assert nfb <= 0.1 * 128
+    for k in ("mean_S0", "mean_d", "mean_dstd", "mean_R", "mean_tau"):
+        rp = np.median(np.abs(plain[k] / oa[k] - 1))
+        rf = np.median(np.abs(ob[k] / oa[k] - 1))
+        print(f"{k}: median relative difference plain GPU vs oracle {rp:.2e}, oracle vs oracle (other seed) {rf:.2e}")
+        assert rp <= 2 * rf + 1e-3
 
 
 def test_learned_start_with_f0(fdt):
