@@ -635,6 +635,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--nvox", type=int, default=0, help="override the voxel count (default: the full mask)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--plain-only", action="store_true",
+                    help="time the plain path only (no end-to-end pass, no learned start): for profiler runs")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--workload", default="c1", choices=["c1", "c2", "c3", "c4", "gradnonlin", "pipeline", "spatial"],
                     help="c1: the multi-shell 3-fibre volume (default); c2: the mixed-acquisition queue; "
@@ -709,16 +711,18 @@ def main():
 
     # the same steps from host buffers: upload of the signal, LM, chains,
     # finishing, download of every output (the PCIe-inclusive rate)
-    x2 = F.Xfibres(**opts)
-    x2.set_acquisition(v["bvals"], v["bvecs"])
-    x2.sync()
-    t = time.perf_counter()
-    for _ in range(args.steps):
-        x2.load_data(v["data"])
-        x2.run(0)
-        x2.outputs()
-    e2e_ms = (time.perf_counter() - t) / args.steps * 1e3
-    x2.close()
+    e2e_ms = float("nan")
+    if not args.plain_only:
+        x2 = F.Xfibres(**opts)
+        x2.set_acquisition(v["bvals"], v["bvecs"])
+        x2.sync()
+        t = time.perf_counter()
+        for _ in range(args.steps):
+            x2.load_data(v["data"])
+            x2.run(0)
+            x2.outputs()
+        e2e_ms = (time.perf_counter() - t) / args.steps * 1e3
+        x2.close()
 
     fl_launch = flops_per_jump(2, 3, N) * (opts["nburn"] + opts["njumps"]) * nvox
     mc_ms = float(np.mean(ms_mcmc))
@@ -757,7 +761,7 @@ def main():
                "flop_model": "N*(np(np+1) + 2np + 40K) FP32 per loop trip at np=12, K=3 (both stages counted at "
                              "the stage-2 size)"},
     }
-    if os.path.exists(SURROGATE_WEIGHTS):
+    if os.path.exists(SURROGATE_WEIGHTS) and not args.plain_only:
         import fdt_amd as F2
         wz = np.load(SURROGATE_WEIGHTS)
         s = F2.Surrogate(N, seed=0)

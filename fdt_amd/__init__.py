@@ -30,7 +30,7 @@ EXPORTS = ["fdt_default_opts", "fdt_create", "fdt_destroy", "fdt_last_error", "f
            "fdt_surrogate_create", "fdt_surrogate_destroy", "fdt_surrogate_last_error", "fdt_surrogate_num_weights",
            "fdt_surrogate_load", "fdt_surrogate_save", "fdt_surrogate_set_training_data", "fdt_surrogate_gradients",
            "fdt_surrogate_train_step", "fdt_surrogate_forward", "fdt_surrogate_infer", "fdt_run_surrogate", "fdt_get_fallback_flags",
-           "fdt_predict", "fdt_set_grid", "fdt_get_spatial_state", "fdt_get_lm_iters", "fdt_mcmc_noalloc"]
+           "fdt_predict", "fdt_set_grid", "fdt_get_spatial_state", "fdt_get_lm_iters", "fdt_mcmc_noalloc", "fdt_set_widths", "fdt_get_energy"]
 
 
 class Opts(C.Structure):
@@ -120,6 +120,8 @@ def lib():
         L.fdt_queue_autotune.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_float)]
         L.fdt_queue_timing.argtypes = [vp, C.POINTER(C.c_float), ip, ip, C.c_int]
         L.fdt_get_widths.argtypes = [vp, _FP]
+        L.fdt_set_widths.argtypes = [vp, _FP]
+        L.fdt_get_energy.argtypes = [vp, _FP]
         L.fdt_surrogate_create.argtypes = [C.c_int, C.c_int, C.c_uint64]
         L.fdt_surrogate_create.restype = vp
         L.fdt_surrogate_destroy.argtypes = [vp]
@@ -298,6 +300,20 @@ class Xfibres:
 
     def set_mcmc_tile(self, vpl):
         self._chk(self.L.fdt_set_mcmc_tile(self.h, vpl))
+
+    def set_widths(self, widths):
+        """initial proposal widths [nvox][16] per slot for the next chains (<= 0: default); None clears"""
+        if widths is None:
+            self._chk(self.L.fdt_set_widths(self.h, None))
+            return
+        w, wp = _f32(np.asarray(widths).reshape(self.nvox, 16))
+        self._chk(self.L.fdt_set_widths(self.h, wp))
+
+    def energy(self):
+        """the likelihood energy each chain carries at its last sweep (Gaussian: the running SSR)"""
+        e = np.zeros(self.nvox, np.float32)
+        self._chk(self.L.fdt_get_energy(self.h, e.ctypes.data_as(_FP)))
+        return e
 
     def widths(self):
         w = np.zeros((self.nvox, 16), np.float32)

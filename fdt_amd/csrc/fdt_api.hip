@@ -188,6 +188,7 @@ struct fdt_ctx {
   DevBuf<float> d_s1;             // finalised stage-1 vectors of the model >= 2 pipeline
   DevBuf<float> d_widths_in, d_widths_out;  // proposal widths per voxel and slot [nvox][16]
   bool use_widths_in = false;
+  bool energy_valid = false;  // d_energy holds the final energy of the last chains
   DevBuf<unsigned char> d_refit;  // voxels refitted without f0
   bool have_params = false, have_samples = false, have_finish = false;
   bool have_iters = false;  // d_iters holds the LM iterations of the current parameters
@@ -251,22 +252,6 @@ static int check_opts(fdt_ctx* c, const fdt_opts& o) {
 static int ensure_acqv(fdt_ctx* c);
 
 extern "C" {
-
-void fdt_default_opts(fdt_opts* o) {
-  std::memset(o, 0, sizeof *o);
-  o->nfibres = 1;
-  o->model = 1;
-  o->fudge = 1.0f;
-  o->njumps = 1250;
-  o->nburn = 1000;
-  o->sampleevery = 25;
-  o->updateproposalevery = 40;
-  o->lm_fp64 = 2;  // FP32 normal equations, FP64 parameters and cost
-  o->seed = 8665904;
-  o->R_prior_mean = 0.13f;
-  o->R_prior_std = 0.03f;
-  o->R_prior_fudge = 0.0f;
-}
 
 fdt_ctx* fdt_create(int device, const fdt_opts* o) {
   if (!o) return nullptr;
@@ -383,6 +368,7 @@ int fdt_load_data(fdt_ctx* c, const float* data, int layout, int nvox) {
   c->nvox = nvox;
   c->acqv_valid = false;
   c->have_params = c->have_samples = c->have_finish = c->have_spatial = false;
+  c->use_widths_in = false;
   return FDT_OK;
 }
 
@@ -870,6 +856,7 @@ static int mcmc_async(fdt_ctx* c, uint64_t vox_offset) {
   if (!c->quiet) HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
   c->timed_mcmc = !c->quiet;
   c->have_samples = true;
+  c->energy_valid = true;
   c->have_finish = false;
   return FDT_OK;
 }
@@ -970,6 +957,7 @@ static int spatial_async(fdt_ctx* c, uint64_t vox_offset) {
   c->timed_fit = false;
   c->timed_mcmc = !c->quiet;
   c->have_samples = true;
+  c->energy_valid = true;
   c->have_finish = false;
   c->have_spatial = true;
   return FDT_OK;
@@ -1247,6 +1235,7 @@ static int queue_issue(fdt_queue* q, bool upload) {
   QHIP(q, hipEventRecord(q->ev_fork, q->stream));
   for (fdt_ctx* c : q->jobs) {
     c->have_samples = true;
+    c->energy_valid = false;
     QHIP(q, hipStreamWaitEvent(c->stream, q->ev_fork, 0));
     QCTX(q, c, finish_async(c));
     QHIP(q, hipEventRecord(c->ev_done, c->stream));
@@ -1842,6 +1831,7 @@ int fdt_run_surrogate(fdt_ctx* c, fdt_surrogate* s, uint64_t vox_offset, int nbu
   HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
   HIPCHK(c, fn(&a, K, o.rician, c->stream));
   c->have_samples = true;
+  c->energy_valid = false;  // d_energy holds the sum-of-squares test's input here
   if (int rc = finish_async(c)) return rc;
   // fallback list
   HIPCHK(c, c->d_queue.ensure(NQUEUE));
@@ -1941,6 +1931,27 @@ int fdt_get_fallback_flags(fdt_ctx* c, unsigned char* flags) {
   if (!c->d_refit.p || c->d_refit.n < (size_t)c->nvox) return set_err(c, FDT_ESTATE, "no learned-start run");
   (void)hipSetDevice(c->device);
   HIPCHK(c, hipMemcpyAsync(flags, c->d_refit.p, c->nvox, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return FDT_OK;
+}
+
+int fdt_set_widths(fdt_ctx* c, const float* widths) {
+  if (!c) return FDT_EINVAL;
+  if (!widths) { c->use_widths_in = false; return FDT_OK; }
+  (void)hipSetDevice(c->device);
+  HIPCHK(c, c->d_widths_in.ensure((size_t)std::max(c->nvox, 1) * 16));
+  if (c->nvox)
+    HIPCHK(c, hipMemcpyAsync(c->d_widths_in.p, widths, sizeof(float) * c->nvox * 16, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->use_widths_in = true;
+  return FDT_OK;
+}
+
+int fdt_get_energy(fdt_ctx* c, float* energy) {
+  if (!c || !energy) return FDT_EINVAL;
+  if (!c->have_samples || !c->energy_valid) return set_err(c, FDT_ESTATE, "no chains have run");
+  (void)hipSetDevice(c->device);
+  HIPCHK(c, hipMemcpyAsync(energy, c->d_energy.p, sizeof(float) * c->nvox, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return FDT_OK;
 }

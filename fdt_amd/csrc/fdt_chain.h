@@ -60,16 +60,29 @@ enum { SL_F0 = 0, SL_TAU = 1, SL_D = 2, SL_DSTD = 3, SL_R = 4, SL_S0 = 5, SL_FIB
 // Signal forms with row-uniform coefficients (compute_signal /
 // compute_iso_signal, CUDA/runmcmc_kernels.cu:67-95):
 //   kind 0: exp(-d*u)                        model 1, or d_std < 1e-5
-//   kind 1: (1 + (dstd^2/d) u)^(-d^2/dstd^2)  model 2 (gamma diffusivities)
+//   kind 1: (1 + w)^(-d^2/dstd^2), w = (dstd^2/d) u   model 2 (gamma diffusivities)
 //   kind 2: exp(-3 d invR ((1-R) b a + R b))  model 3 zeppelin, u = b a
+// Kind 1 in FP32: u1 = fl(1 + w) keeps w only to 2^-24 absolute, and the
+// exponent d^2/dstd^2 multiplies that error (1e-3 relative signal error at
+// dstd/d ~ 0.01; the reference evaluates these signals in double).  The
+// rounding error e = (1 + w) - u1 is recovered with one FMA and
+// lg2(1 + w) = lg2(u1) + lg2(1 + e/u1) ~ lg2(u1) + e/ln2; the dropped 1/u1
+// factor leaves an exponent error below d u 2^-24 for every w (e <= half an
+// ulp of u1, and (d^2/dstd^2) w = d u).
 struct SigC {
   int kind;
   float c0, c1;
 };
+FDT_DEV SigC gamma_coef(float d, float dstd) {
+  SigC c;
+  const float s2 = dstd * dstd;
+  c.kind = 1; c.c0 = s2 / d; c.c1 = -(d * d / s2);
+  return c;
+}
 FDT_DEV SigC aniso_coef(int model, float d, float dstd, float R) {
   SigC c;
   if (model == 1 || (model == 2 && dstd < 1e-5f)) { c.kind = 0; c.c0 = -d * L2E; c.c1 = 0.0f; }
-  else if (model == 2) { float s2 = dstd * dstd; c.kind = 1; c.c0 = s2 / d; c.c1 = -(d * d / s2); }
+  else if (model == 2) c = gamma_coef(d, dstd);
   else {
     float iR = 1.0f / (2.0f * R + 1.0f);
     float k = -3.0f * d * iR * L2E;
@@ -80,13 +93,17 @@ FDT_DEV SigC aniso_coef(int model, float d, float dstd, float R) {
 FDT_DEV SigC iso_coef(int model, float d, float dstd) {
   SigC c;
   if (model == 1 || dstd < 1e-5f) { c.kind = 0; c.c0 = -d * L2E; c.c1 = 0.0f; }
-  else { float s2 = dstd * dstd; c.kind = 1; c.c0 = s2 / d; c.c1 = -(d * d / s2); }
+  else c = gamma_coef(d, dstd);
   return c;
 }
 template <int SK>
 FDT_DEV float sig(const SigC& c, float u, float b) {
   if constexpr (SK == 0) return ex2(c.c0 * u);
-  else if constexpr (SK == 1) return ex2(c.c1 * lg2(fmaf(c.c0, u, 1.0f)));
+  else if constexpr (SK == 1) {
+    const float u1 = fmaf(c.c0, u, 1.0f);
+    const float e = fmaf(c.c0, u, 1.0f - u1);  // (1 + w) - u1 (1 - u1 is exact)
+    return ex2(c.c1 * fmaf(e, L2E, lg2(u1)));
+  }
   else return ex2(fmaf(c.c0, u, c.c1 * b));
 }
 template <class F>
@@ -299,7 +316,7 @@ struct Chain {
         ss += (double)(r[t] * r[t]);
       }
     }
-    like = (double)row_sum16((float)ss);
+    like = row_sum16_d(ss);
   }
 
   // ---------------------------------------------------------- proposals
@@ -491,6 +508,9 @@ struct Chain {
                              decltype(SH)::value>(t, q, ca, ci, D);
           if constexpr (RIC) a += mask(t, ell_of(r[t], S0 * mn, tau) - ell[t]);
           else qd.add(r[t], mask(t, mn - M[t]));
+          // the gamma form has several temporaries per volume and signal: keep
+          // the scheduler from overlapping many volumes of it (register spills)
+          if constexpr (decltype(SA)::value == 1 || decltype(SI)::value == 1) reload_acq();
         }
       });
     });
@@ -826,7 +846,24 @@ FDT_DEV void run_chain(const McmcArgs& a, int vox, const ChainLds& L) {
         if (smp < a.nsamples && rec_base) rec_base[(size_t)smp * rec_stride] = c.sval;
       }
     }
-    if ((j + 1) % a.upe == 0) c.update_proposals();
+    if ((j + 1) % a.upe == 0) {
+      c.update_proposals();
+      // Re-anchor the incremental state with the widths: the mixtures, the
+      // residuals (Gaussian: from the data, re-read) and the running
+      // likelihood recomputed from the stored signals, so rounding in the
+      // per-move updates cannot accumulate over the whole chain (the
+      // reference recomputes its likelihood at every proposal).
+      if constexpr (RIC) c.rebuild(nullptr);
+      else {
+        float yr[VPL];
+#pragma unroll
+        for (int t = 0; t < VPL; ++t) {
+          const int i = rl + 16 * t;
+          yr[t] = i < a.N ? yrow[i] : 0.0f;
+        }
+        c.rebuild(yr);
+      }
+    }
   }
   if (a.energy) {
     float tau = c.template v<SL_TAU>();

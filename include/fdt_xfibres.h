@@ -269,6 +269,20 @@ int fdt_queue_timing(fdt_queue* q, float* ms_last, int* rows_per_wg, int* vpl_of
  * CUDA/fibre_gpu.h:12-83); they label the training set of the learned start. */
 int fdt_get_widths(fdt_ctx* c, float* widths);
 
+/* Initial proposal widths for the next chains: widths [nvox][16] per voxel and
+ * slot (layout of fdt_get_widths); a value <= 0 keeps the default width of
+ * init_Fibres_Multifibres_kernel (CUDA/runmcmc_kernels.cu:497-503).  NULL
+ * clears them.  Cleared by fdt_load_data; the learned start
+ * (fdt_run_surrogate) sets its own. */
+int fdt_set_widths(fdt_ctx* c, const float* widths);
+
+/* Per voxel, the likelihood energy each chain carries at its last sweep of
+ * the last fdt_mcmc / fdt_run: Gaussian noise: the running sum of squared
+ * residuals; Rician: -N log tau minus the summed log-likelihood terms (the
+ * likelihood part of compute_likelihood, CUDA/runmcmc_kernels.cu:147-191).
+ * The chains update it incrementally; exposed to bound that drift. */
+int fdt_get_energy(fdt_ctx* c, float* energy);
+
 /* ------------------------------------------------------ learned start
  * An MLP N -> 512 -> 512 -> 256 -> 28 (GELU, BF16 weights and activations,
  * FP32 accumulation) maps a voxel's signal over its mean b=0 signal to start

@@ -63,7 +63,7 @@ def lib():
             getattr(L, p + "_start").argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, dp, dp, C.c_int, dp, dp, dp]
             getattr(L, p + "_fit").argtypes = [C.POINTER(OrcOpts), C.c_int, dp, dp, C.c_int, dp, dp, dp, ip, dp]
             getattr(L, p + "_mcmc").argtypes = [C.POINTER(OrcOpts), C.c_int, dp, dp, C.c_int, dp, dp, dp,
-                                               C.c_uint64, C.POINTER(OrcSamples), dp, dp]
+                                               C.c_uint64, C.POINTER(OrcSamples), dp, dp, dp]
         L.orc_dti.argtypes = [C.c_int, dp, dp, C.c_int, dp, dp]
         L.orc_correct_acq.argtypes = [C.c_int, dp, dp, dp, dp, dp]
         L.orc_pinv.argtypes = [C.c_int, C.c_int, dp, dp]
@@ -241,7 +241,9 @@ def alloc_samples(o: OrcOpts, nvox: int):
     return arrs, s
 
 
-def mcmc(o: OrcOpts, bvals, bvecs, data, params, tau=None, vox_offset=0, prec=64, grad=None):
+def mcmc(o: OrcOpts, bvals, bvecs, data, params, tau=None, vox_offset=0, prec=64, grad=None, widths=None):
+    """chains from the start vectors `params`; widths [nvox][16]: initial
+    proposal widths per draw slot (<= 0 keeps the default), or None"""
     data = np.ascontiguousarray(data, dtype=np.float64)
     nvox, N = data.shape
     params = np.ascontiguousarray(params, dtype=np.float64)
@@ -251,7 +253,8 @@ def mcmc(o: OrcOpts, bvals, bvecs, data, params, tau=None, vox_offset=0, prec=64
     bv, bc = _d(bvals), _d(bvecs)
     getattr(lib(), f"orc{prec}_mcmc")(C.byref(o), N, bv[1], bc[1], nvox, data.ctypes.data_as(C.POINTER(C.c_double)),
                                       params.ctypes.data_as(C.POINTER(C.c_double)), tp, vox_offset, C.byref(s),
-                                      energy.ctypes.data_as(C.POINTER(C.c_double)), _grad(grad, nvox)[1])
+                                      energy.ctypes.data_as(C.POINTER(C.c_double)), _grad(grad, nvox)[1],
+                                      _d(widths)[1] if widths is not None else C.POINTER(C.c_double)())
     arrs["energy"] = energy
     return arrs
 

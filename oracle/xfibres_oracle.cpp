@@ -1281,6 +1281,14 @@ struct Chain {
     }
   }
 
+  // initial proposal widths per draw slot (<= 0: keep the default); the
+  // product's fdt_set_widths, used by the learned start
+  void set_widths(const double* w) {
+    auto put = [&](R& prop, int slot) { if (w[slot] > 0) prop = (R)w[slot]; };
+    put(f0_prop, 0); put(tau_prop, 1); put(d_prop, 2); put(dstd_prop, 3); put(R_prop, 4); put(S0_prop, 5);
+    for (int k = 0; k < K; ++k) { put(th_prop[k], 6 + 3 * k); put(ph_prop[k], 7 + 3 * k); put(f_prop[k], 8 + 3 * k); }
+  }
+
   static void upd(R& prop, int& acc, int& rej) {  // :1345-1393
     prop *= std::sqrt(R(acc + 1) / R(rej + 1));
     prop = std::min(prop, (R)MAXFLOAT_C);
@@ -1410,7 +1418,7 @@ typedef struct orc_samples {
   }                                                                                                           \
   int PFX##_mcmc(const orc_opts* o, int N, const double* bvals, const double* bvecs, int nvox,                \
                  const double* data, const double* params, const double* tau, uint64_t vox_offset,            \
-                 orc_samples* out, double* final_energy, const double* grad) {                                \
+                 orc_samples* out, double* final_energy, const double* grad, const double* widths) {          \
     orc::Acq<RT> acq0(N, bvals, bvecs);                                                                        \
     int K = o->nfibres;                                                                                       \
     int npo = 2 + 3 * K + (o->model >= 2 ? 1 : 0) + (o->f0 ? 1 : 0);                                          \
@@ -1426,6 +1434,7 @@ typedef struct orc_samples {
       orc::VoxAcq<RT> va(N, bvals, bvecs, grad ? grad + (size_t)v * 9 : nullptr, acq0, none, false);            \
       orc::Chain<RT> ch(*o, *va.acq, y.data());                                                                \
       ch.init(p.data(), tau ? (RT)tau[v] : RT(0));                                                              \
+      if (widths) ch.set_widths(widths + (size_t)v * 16);                                                      \
       uint64_t gv = vox_offset + (uint64_t)v;                                                                 \
       int total = o->nburn + o->njumps;                                                                       \
       for (int j = 0; j < total; ++j) {                                                                       \

@@ -93,6 +93,11 @@ def test_posterior_parity_with_oracle(orc, fdt, name, model, K, nvox):
         assert cross[m] >= floor[m] - PS.tol(floor[m], n), (m, cross[m], floor[m])
         if prec[m] >= PS.BAR[m]:
             assert track[m] >= PS.BAR[m] - PS.tol(PS.BAR[m], n), (m, track[m])
+    for k in ("mean_S0", "mean_d", "mean_dstd"):
+        if k in g and np.any(o[k]):
+            print(f"{k}: median relative difference to the oracle: GPU {np.median(np.abs(g[k] / o[k] - 1)):.2e} "
+                  f"(same seed), {np.median(np.abs(g[k] / ob[k] - 1)):.2e} (other seed); oracle other seed "
+                  f"{np.median(np.abs(ob[k] / o[k] - 1)):.2e}")
     s0 = np.abs(g["mean_S0"] / o["mean_S0"] - 1)
     s0f = np.abs(ob["mean_S0"] / o["mean_S0"] - 1)
     print(f"mean S0 within 2% of the oracle: GPU {np.mean(s0 <= 0.02):.3f}, oracle other seed {np.mean(s0f <= 0.02):.3f}")
