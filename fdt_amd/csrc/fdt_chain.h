@@ -59,8 +59,12 @@ enum { SL_F0 = 0, SL_TAU = 1, SL_D = 2, SL_DSTD = 3, SL_R = 4, SL_S0 = 5, SL_FIB
 
 // Signal forms with row-uniform coefficients (compute_signal /
 // compute_iso_signal, CUDA/runmcmc_kernels.cu:67-95):
-//   kind 0: exp(-d*u)                        model 1, or d_std < 1e-5
-//   kind 1: (1 + w)^(-d^2/dstd^2), w = (dstd^2/d) u   model 2 (gamma diffusivities)
+//   kind 0: exp(-d*u)                        model 1
+//   kind 1: (1 + w)^(-d^2/dstd^2), w = (dstd^2/d) u   model 2 (gamma diffusivities);
+//           below d_std 1e-5 the reference switches to exp(-d*u), which the
+//           same expression gives exactly with c0 = c1 = 0 and c2 = -d/ln2,
+//           so rows on either side of the switch share one code path (a
+//           wave whose rows took different forms would run both)
 //   kind 2: exp(-3 d invR ((1-R) b a + R b))  model 3 zeppelin, u = b a
 // Kind 1 in FP32: u1 = fl(1 + w) keeps w only to 2^-24 absolute, and the
 // exponent d^2/dstd^2 multiplies that error (1e-3 relative signal error at
@@ -71,28 +75,29 @@ enum { SL_F0 = 0, SL_TAU = 1, SL_D = 2, SL_DSTD = 3, SL_R = 4, SL_S0 = 5, SL_FIB
 // ulp of u1, and (d^2/dstd^2) w = d u).
 struct SigC {
   int kind;
-  float c0, c1;
+  float c0, c1, c2;
 };
 FDT_DEV SigC gamma_coef(float d, float dstd) {
   SigC c;
-  const float s2 = dstd * dstd;
-  c.kind = 1; c.c0 = s2 / d; c.c1 = -(d * d / s2);
+  c.kind = 1;
+  if (dstd < 1e-5f) { c.c0 = 0.0f; c.c1 = 0.0f; c.c2 = -d * L2E; }
+  else { const float s2 = dstd * dstd; c.c0 = s2 / d; c.c1 = -(d * d / s2); c.c2 = 0.0f; }
   return c;
 }
 FDT_DEV SigC aniso_coef(int model, float d, float dstd, float R) {
   SigC c;
-  if (model == 1 || (model == 2 && dstd < 1e-5f)) { c.kind = 0; c.c0 = -d * L2E; c.c1 = 0.0f; }
+  if (model == 1) { c.kind = 0; c.c0 = -d * L2E; c.c1 = 0.0f; c.c2 = 0.0f; }
   else if (model == 2) c = gamma_coef(d, dstd);
   else {
     float iR = 1.0f / (2.0f * R + 1.0f);
     float k = -3.0f * d * iR * L2E;
-    c.kind = 2; c.c0 = k * (1.0f - R); c.c1 = k * R;
+    c.kind = 2; c.c0 = k * (1.0f - R); c.c1 = k * R; c.c2 = 0.0f;
   }
   return c;
 }
 FDT_DEV SigC iso_coef(int model, float d, float dstd) {
   SigC c;
-  if (model == 1 || dstd < 1e-5f) { c.kind = 0; c.c0 = -d * L2E; c.c1 = 0.0f; }
+  if (model == 1) { c.kind = 0; c.c0 = -d * L2E; c.c1 = 0.0f; c.c2 = 0.0f; }
   else c = gamma_coef(d, dstd);
   return c;
 }
@@ -102,7 +107,7 @@ FDT_DEV float sig(const SigC& c, float u, float b) {
   else if constexpr (SK == 1) {
     const float u1 = fmaf(c.c0, u, 1.0f);
     const float e = fmaf(c.c0, u, 1.0f - u1);  // (1 + w) - u1 (1 - u1 is exact)
-    return ex2(c.c1 * fmaf(e, L2E, lg2(u1)));
+    return ex2(fmaf(c.c1, fmaf(e, L2E, lg2(u1)), c.c2 * u));
   }
   else return ex2(fmaf(c.c0, u, c.c1 * b));
 }
@@ -437,7 +442,7 @@ struct Chain {
     }
   }
   // dispatch over the coefficient forms that occur: model 1 (0,0); model 2
-  // (1,1) or (0,0) below d_std 1e-5; model 3 sticks 2 with iso 1 or 0
+  // (1,1); model 3 sticks 2 with iso 1
   // run f(SH) with SH = true when the isotropic signal changes and the
   // acquisition has a shell table (the loop over slots is then the table
   // variant only; no per-slot branch)
@@ -455,14 +460,13 @@ struct Chain {
     using I2 = std::integral_constant<int, 2>;
     using T_ = std::true_type;
     using F_ = std::false_type;
+    // the isotropic form is kind 0 for model 1 and kind 1 otherwise
     if (na && ni) {
       if (ca.kind == 0) f(I0{}, I0{}, T_{}, T_{});
       else if (ca.kind == 1) f(I1{}, I1{}, T_{}, T_{});
-      else if (ci.kind == 1) f(I2{}, I1{}, T_{}, T_{});
-      else f(I2{}, I0{}, T_{}, T_{});
+      else f(I2{}, I1{}, T_{}, T_{});
     } else if (ni) {
-      if (ci.kind == 1) f(I0{}, I1{}, F_{}, T_{});
-      else f(I0{}, I0{}, F_{}, T_{});
+      f(I0{}, I1{}, F_{}, T_{});
     } else {
       f(I2{}, I0{}, T_{}, F_{});
     }
