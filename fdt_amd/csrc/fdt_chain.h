@@ -190,7 +190,11 @@ struct Chain {
   float sval, sprop, sprior;
   int scnt;
   // sin and cos of the slot's angle (theta_k / phi_k slots), kept with the
-  // value so that fibre directions are row broadcasts, not re-evaluated
+  // value so that fibre directions are row broadcasts, not re-evaluated.
+  // The d and d_std slots hold the stick-signal coefficients of the current
+  // state there instead (c0, c1 in the d lane, c2 in the d_std lane), set
+  // whenever d, d_std or R moves, so that the angle proposals broadcast them
+  // instead of dividing again.
   float ssin, scos;
   double like;  // Gaussian: SSR; Rician: sum of ell
   int nv;       // slots of this lane that hold a real volume (rl + 16 t < N)
@@ -231,6 +235,19 @@ struct Chain {
   }
 
   FDT_DEV float& cnd(int k, int t) const { return cand[(k * VPL + t) * 64]; }
+  // stick-signal coefficients of the current state (the kind follows the model)
+  FDT_DEV SigC aniso_cached() const {
+    SigC c;
+    c.kind = model == 1 ? 0 : (model == 2 ? 1 : 2);
+    c.c0 = row_bcast<SL_D>(ssin);
+    c.c1 = row_bcast<SL_D>(scos);
+    c.c2 = row_bcast<SL_DSTD>(ssin);
+    return c;
+  }
+  FDT_DEV void set_aniso_cache(const SigC& c) {
+    if (rl == SL_D) { ssin = c.c0; scos = c.c1; }
+    if (rl == SL_DSTD) ssin = c.c2;
+  }
   // unit directions of the row's fibres (from the theta / phi slots)
   struct Dirs {
     float x[K][3];
@@ -346,11 +363,12 @@ struct Chain {
     }
     a = RIC ? row_sum16(a) : qd.delta(S0 * df);
     if (dlik(a) + (prn - prior<SL_F0>()) < -lu) {
+      const float c = S0 * df;
 #pragma unroll
       for (int t = 0; t < VPL; ++t) {
-        float dM = df * (1.0f - iso[t]);
-        M[t] += dM;
-        if constexpr (RIC) ell[t] = en[t]; else r[t] -= S0 * dM;
+        const float q = 1.0f - iso[t];
+        M[t] = fmaf(df, q, M[t]);
+        if constexpr (RIC) ell[t] = en[t]; else r[t] = fmaf(-c, q, r[t]);
       }
       like += (double)a;
       accept<SL_F0>(f0n, prn);
@@ -537,6 +555,7 @@ struct Chain {
       });
       like += (double)a;
       accept<SL>(xn, prn);
+      set_aniso_cache(ca);
     } else reject<SL>();
   }
   // theta_k / phi_k (:1157-1265); the theta prior lives in the theta slot
@@ -554,7 +573,7 @@ struct Chain {
     const float sp = IS_TH ? so : sn_, cp = IS_TH ? co : cn_;
     float x0 = st * cp, x1 = st * sp, x2 = ct;
     xform(x0, x1, x2);
-    const SigC ca = aniso_coef(model, v<SL_D>(), v<SL_DSTD>(), v<SL_R>());
+    const SigC ca = aniso_cached();
     const float fk = v<FK(k)>(), S0 = v<SL_S0>(), tau = v<SL_TAU>();
     reload_acq();
     float sn[VPL], en[RIC ? VPL : 1];
@@ -578,11 +597,12 @@ struct Chain {
     a = RIC ? row_sum16(a) : qd.delta(S0 * fk);
     if (dlik(a) + (prn - prold) < -lu) {
       reload_acq();
+      const float c = S0 * fk;
 #pragma unroll
       for (int t = 0; t < VPL; ++t) {
-        float dM = fk * (sn[t] - s[k][t]);
-        M[t] += dM;
-        if constexpr (RIC) ell[t] = en[t]; else r[t] -= S0 * dM;
+        const float q = sn[t] - s[k][t];
+        M[t] = fmaf(fk, q, M[t]);
+        if constexpr (RIC) ell[t] = en[t]; else r[t] = fmaf(-c, q, r[t]);
         s[k][t] = sn[t];
       }
       like += (double)a;
@@ -612,11 +632,12 @@ struct Chain {
     }
     a = RIC ? row_sum16(a) : qd.delta(S0 * df);
     if (dlik(a) + (prn - prior<SL>()) < -lu) {
+      const float c = S0 * df;
 #pragma unroll
       for (int t = 0; t < VPL; ++t) {
-        float dM = df * (s[k][t] - iso[t]);
-        M[t] += dM;
-        if constexpr (RIC) ell[t] = en[t]; else r[t] -= S0 * dM;
+        const float q = s[k][t] - iso[t];
+        M[t] = fmaf(df, q, M[t]);
+        if constexpr (RIC) ell[t] = en[t]; else r[t] = fmaf(-c, q, r[t]);
       }
       like += (double)a;
       accept<SL>(fn, prn);
@@ -783,6 +804,7 @@ FDT_DEV void run_chain(const McmcArgs& a, int vox, const ChainLds& L) {
   c.scnt = 0;
   {
     SigC ca = aniso_coef(a.model, d, dstd, Rr), ci = iso_coef(a.model, d, dstd);
+    c.set_aniso_cache(ca);
     const typename C::Dirs D = c.dirs();
     by_kind(ca.kind, [&](auto SK) {
 #pragma unroll

@@ -18,9 +18,12 @@ constexpr float MAXPROP_F = 1e10f;     // fibre.h:19
 // All-reduce inside a 16-lane DPP row: quad_perm xor1, quad_perm xor2,
 // row_half_mirror, row_mirror.  Every lane of the row ends with the same sum
 // (each step adds a partner value, and float addition is commutative).
+// DPP lane moves with bound_ctrl set and every row and bank enabled: every
+// control used here reads a lane inside the row, so no lane keeps an old
+// value and the compiler needs no register initialisation for it
 template <int CTRL>
 FDT_DEV float dpp(float v) {
-  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, 0xF, 0xF, false));
+  return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), CTRL, 0xF, 0xF, true));
 }
 FDT_DEV float row_sum16(float v) {
   v += dpp<0xB1>(v);   // quad_perm [1,0,3,2]
@@ -70,8 +73,8 @@ FDT_DEV float wave_min(float v) {
 template <int CTRL>
 FDT_DEV double dppd(double v) {
   int2 b = __builtin_bit_cast(int2, v);
-  b.x = __builtin_amdgcn_update_dpp(0, b.x, CTRL, 0xF, 0xF, false);
-  b.y = __builtin_amdgcn_update_dpp(0, b.y, CTRL, 0xF, 0xF, false);
+  b.x = __builtin_amdgcn_mov_dpp(b.x, CTRL, 0xF, 0xF, true);
+  b.y = __builtin_amdgcn_mov_dpp(b.y, CTRL, 0xF, 0xF, true);
   return __builtin_bit_cast(double, b);
 }
 FDT_DEV double rld(double v, int lane) {
@@ -111,7 +114,7 @@ FDT_DEV double row_bcast_d(double v) {
 template <int L>
 FDT_DEV int row_bcast_i(int v) {
   static_assert(L >= 0 && L < 16, "row lane");
-  return __builtin_amdgcn_update_dpp(0, v, 0x150 + L, 0xF, 0xF, false);
+  return __builtin_amdgcn_mov_dpp(v, 0x150 + L, 0xF, 0xF, true);
 }
 // value of v in lane `src` of the wave (ds_bpermute; src may differ per lane)
 FDT_DEV double bperm_d(int src, double v) {
@@ -150,16 +153,21 @@ FDT_DEV u4 philox4x32_10(u4 c, uint32_t k0, uint32_t k1) {
   return c;
 }
 // Standard normal by Box-Muller (cosine branch) and log of an acceptance
-// uniform in (0,1], from one Philox block (24-bit uniforms).
+// uniform in (0,1], from one Philox block (24-bit uniforms).  With the
+// hardware log2 and cos (v_cos_f32 takes its argument in revolutions, so
+// cos(2 pi u2) needs no reduction and no rounding of 2 pi u2) the normal is
+// within 7.6e-7 of the double-precision mapping and log u within 1.7e-7
+// relative, over all 2^24 values of u2 (libm's float forms: 1.7e-6, from
+// rounding 2 pi u2 before the cosine).
 FDT_DEV void draw(uint64_t seed, uint64_t vox, uint32_t jump, uint32_t slot, float& n, float& logu) {
   u4 c{(uint32_t)vox, (uint32_t)(vox >> 32), jump, slot};
   u4 x = philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
-  const float s24 = 1.0f / 16777216.0f;
+  const float s24 = 1.0f / 16777216.0f, LN2 = 0.69314718055994531f;
   float u1 = (float)((x.x >> 8) + 1u) * s24;
   float u2 = (float)(x.y >> 8) * s24;
   float u3 = (float)((x.z >> 8) + 1u) * s24;
-  n = sqrtf(-2.0f * logf(u1)) * cosf(6.283185307179586f * u2);
-  logu = logf(u3);
+  n = sqrtf(-2.0f * LN2 * __builtin_amdgcn_logf(u1)) * __builtin_amdgcn_cosf(u2);
+  logu = LN2 * __builtin_amdgcn_logf(u3);
 }
 
 // ----------------------------------------------------------- transforms
