@@ -32,7 +32,7 @@ FP32_PEAK_TFLOPS = 157.3
 # HBM bytes per launch of mcmc_kernel<3,18,false> on the full 250,968-voxel
 # workload: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE, each scaled by the same
 # counter's reading for a copy kernel of known size (profiles/mcmc_c1_traffic.md)
-MCMC_TRAFFIC_BYTES = (2 * 148874 + 686986) * 1024
+MCMC_TRAFFIC_BYTES = (2 * 198769 + 835434) * 1024
 METRIC = "voxels/s, 3-fibre multi-shell bedpostx; surrogate vs plain at equal dyad error"
 LM_MODES = {0: "FP32 model, Jacobian and cost", 1: "FP64 throughout",
             2: "FP32 Jacobian and normal equations, FP64 parameters and cost"}
@@ -381,7 +381,7 @@ def bench_gradnonlin(args):
         "config": {"workload": f"multi-shell model 2, nf=3, N={N}, {nvox} voxels acquired through per-voxel "
                                "gradient deviations (grad_dev ~ N(0, 0.03)), --gradnonlin, bi=1000 nj=1250 se=25",
                    "ms_fit": g["ms_fit"], "ms_mcmc": g["ms_mcmc"], "s_synth": t_syn},
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None, "kernel": "mcmc_kernel<3,18,false,true>",
                      "flops_per_launch": fl_launch},
         "shared_acquisition": {"voxels_per_s": p["rate"], "ms_fit": p["ms_fit"], "ms_mcmc": p["ms_mcmc"]},
@@ -773,9 +773,11 @@ def main():
         sg50["speedup_vs_plain"] = sg50["value"] / value
         sg["burnin50"] = sg50
         sg["ablation_plain_lm_short_burnin"] = {str(bi): plain_short_burnin(v, opts, bi) for bi in (200, 50)}
-        sg["speedup_bound"] = ("with the 1,250 recorded sweeps kept, the chains alone cost "
-                               f"{opts['njumps'] / (opts['nburn'] + opts['njumps']):.2f} of the plain step's chain "
-                               "time, so no start can exceed ~2.05x over the plain path here")
+        rec = opts["njumps"] / (opts["nburn"] + opts["njumps"])
+        floor_ms = mc_ms * rec + float(np.mean(ms_fin))
+        sg["speedup_bound"] = (f"with the {opts['njumps']:,} recorded sweeps kept, a pass costs at least their chain "
+                               f"time ({rec:.2f} of the plain chain launch) plus finishing, {floor_ms:.0f} ms, so no "
+                               f"start can exceed {ms_step / floor_ms:.2f}x over the plain path here")
         sg["weights"] = (f"fdt_amd/weights/{os.path.basename(SURROGATE_WEIGHTS)}: trained by bench.py --workload c3 "
                          f"on {int(wz['train_voxels'])} voxels, {int(wz['epochs'])} epochs")
         line["surrogate"] = sg
