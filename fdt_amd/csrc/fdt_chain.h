@@ -174,11 +174,18 @@ FDT_DEV void sfor(F&& f) {
 // voxel's own b values, kept per row in LDS.
 template <int K, int VPL, bool RIC, bool GN = false>
 struct Chain {
-  // lane-resident volume state, volume i = rl + 16 t
-  float r[VPL];  // Gaussian: y - S0*M; Rician: y
+  // lane-resident volume state, volume i = rl + 16 t.  One per-volume
+  // array lives in the wave's LDS block `vol` instead of registers (18
+  // VGPRs at 18 volumes per lane, the difference between spilling and not):
+  // the isotropic signal of the Gaussian chain, the data of the Rician one.
+  // The --gradnonlin kernels keep both in registers (their LDS holds the
+  // rows' b values).
+  static constexpr bool ISO_LDS = !GN && !RIC, Y_LDS = !GN && RIC;
+  float r[Y_LDS ? 1 : VPL];  // Gaussian: y - S0*M; Rician: y
   float M[VPL];
-  float iso[VPL];
+  float iso_r[ISO_LDS ? 1 : VPL];
   float s[K][VPL];
+  float* vol;   // wave's LDS block: vol[t*64]
   float* cand;  // candidate stick signals of the diffusivity proposals, wave's LDS block: cand[(k*VPL+t)*64]
   float ell[RIC ? VPL : 1];
   // isotropic signal per b-shell (acquisitions with <= 16 distinct b values):
@@ -235,6 +242,19 @@ struct Chain {
   }
 
   FDT_DEV float& cnd(int k, int t) const { return cand[(k * VPL + t) * 64]; }
+  FDT_DEV float& isov(int t) {
+    if constexpr (ISO_LDS) return vol[t * 64];
+    else return iso_r[t];
+  }
+  FDT_DEV float isov(int t) const {
+    if constexpr (ISO_LDS) return vol[t * 64];
+    else return iso_r[t];
+  }
+  // the Rician chain's data of slot t
+  FDT_DEV float yv(int t) const {
+    if constexpr (Y_LDS) return vol[t * 64];
+    else return r[t];
+  }
   // stick-signal coefficients of the current state (the kind follows the model)
   FDT_DEV SigC aniso_cached() const {
     SigC c;
@@ -315,7 +335,7 @@ struct Chain {
   }
   // mixture of slot t from the stored signals
   FDT_DEV float mix(int t, const Fr& q) const {
-    float m = fmaf(1.0f - q.sum, iso[t], q.f0);
+    float m = fmaf(1.0f - q.sum, isov(t), q.f0);
 #pragma unroll
     for (int k = 0; k < K; ++k) m = fmaf(q.fk[k], s[k][t], m);
     return m;
@@ -331,7 +351,7 @@ struct Chain {
       float m = mix(t, q);
       M[t] = m;
       if constexpr (RIC) {
-        ell[t] = ell_of(r[t], S0 * m, tau);
+        ell[t] = ell_of(yv(t), S0 * m, tau);
         ss += (double)mask(t, ell[t]);
       } else {
         r[t] = mask(t, y[t] - S0 * m);
@@ -357,16 +377,16 @@ struct Chain {
 #pragma unroll
     for (int t = 0; t < VPL; ++t) {
       if constexpr (RIC) {
-        en[t] = ell_of(r[t], S0 * fmaf(df, 1.0f - iso[t], M[t]), tau);
+        en[t] = ell_of(yv(t), S0 * fmaf(df, 1.0f - isov(t), M[t]), tau);
         a += mask(t, en[t] - ell[t]);
-      } else qd.add(r[t], 1.0f - iso[t]);
+      } else qd.add(r[t], 1.0f - isov(t));
     }
     a = RIC ? row_sum16(a) : qd.delta(S0 * df);
     if (dlik(a) + (prn - prior<SL_F0>()) < -lu) {
       const float c = S0 * df;
 #pragma unroll
       for (int t = 0; t < VPL; ++t) {
-        const float q = 1.0f - iso[t];
+        const float q = 1.0f - isov(t);
         M[t] = fmaf(df, q, M[t]);
         if constexpr (RIC) ell[t] = en[t]; else r[t] = fmaf(-c, q, r[t]);
       }
@@ -383,7 +403,7 @@ struct Chain {
       const float S0 = v<SL_S0>();
       float a = 0.0f, en[VPL];
 #pragma unroll
-      for (int t = 0; t < VPL; ++t) { en[t] = ell_of(r[t], S0 * M[t], tn); a += mask(t, en[t] - ell[t]); }
+      for (int t = 0; t < VPL; ++t) { en[t] = ell_of(yv(t), S0 * M[t], tn); a += mask(t, en[t] - ell[t]); }
       a = row_sum16(a);
       float dE = -Nf * (flog(tn) - flog(tau)) - a;
       if (dE < -lu) {
@@ -404,7 +424,7 @@ struct Chain {
     Quad qd;
 #pragma unroll
     for (int t = 0; t < VPL; ++t) {
-      if constexpr (RIC) { en[t] = ell_of(r[t], Sn * M[t], tau); a += mask(t, en[t] - ell[t]); }
+      if constexpr (RIC) { en[t] = ell_of(yv(t), Sn * M[t], tau); a += mask(t, en[t] - ell[t]); }
       else qd.add(r[t], mask(t, M[t]));
     }
     a = RIC ? row_sum16(a) : qd.delta(dS);
@@ -436,7 +456,7 @@ struct Chain {
   template <int SA, int SI, bool NA, bool NI, bool SH>
   FDT_DEV float mix_new(int t, const Fr& q, const SigC& ca, const SigC& ci, const Dirs& D) const {
     const float4 g = acq[rl + 16 * t];
-    float isn = NI ? iso_new<SI, SH>(t, ci, bv(t, g)) : iso[t];
+    float isn = NI ? iso_new<SI, SH>(t, ci, bv(t, g)) : isov(t);
     float m = fmaf(1.0f - q.sum, isn, q.f0);
 #pragma unroll
     for (int k = 0; k < K; ++k) {
@@ -453,7 +473,7 @@ struct Chain {
   template <int SA, int SI, bool NA, bool NI, bool SH>
   FDT_DEV void store_new(int t, const SigC& ca, const SigC& ci) {
     float b = bv(t, acq[rl + 16 * t]);
-    if (NI) iso[t] = iso_new<SI, SH>(t, ci, b);
+    if (NI) isov(t) = iso_new<SI, SH>(t, ci, b);
     if (NA) {
 #pragma unroll
       for (int k = 0; k < K; ++k) s[k][t] = cnd(k, t);
@@ -528,7 +548,7 @@ struct Chain {
         for (int t = 0; t < VPL; ++t) {
           float mn = mix_new<decltype(SA)::value, decltype(SI)::value, decltype(NA)::value, decltype(NI)::value,
                              decltype(SH)::value>(t, q, ca, ci, D);
-          if constexpr (RIC) a += mask(t, ell_of(r[t], S0 * mn, tau) - ell[t]);
+          if constexpr (RIC) a += mask(t, ell_of(yv(t), S0 * mn, tau) - ell[t]);
           else qd.add(r[t], mask(t, mn - M[t]));
           // the gamma form has several temporaries per volume and signal: keep
           // the scheduler from overlapping many volumes of it (register spills)
@@ -548,7 +568,7 @@ struct Chain {
             store_new<decltype(SA)::value, decltype(SI)::value, decltype(NA)::value, decltype(NI)::value,
                       decltype(SH)::value>(t, ca, ci);
             float mn = mix(t, q);
-            if constexpr (RIC) ell[t] = ell_of(r[t], S0 * mn, tau); else r[t] -= S0 * mask(t, mn - M[t]);
+            if constexpr (RIC) ell[t] = ell_of(yv(t), S0 * mn, tau); else r[t] -= S0 * mask(t, mn - M[t]);
             M[t] = mn;
           }
         });
@@ -590,7 +610,7 @@ struct Chain {
 #pragma unroll
     for (int t = 0; t < VPL; ++t) {
       if constexpr (RIC) {
-        en[t] = ell_of(r[t], S0 * fmaf(fk, sn[t] - s[k][t], M[t]), tau);
+        en[t] = ell_of(yv(t), S0 * fmaf(fk, sn[t] - s[k][t], M[t]), tau);
         a += mask(t, en[t] - ell[t]);
       } else qd.add(r[t], sn[t] - s[k][t]);
     }
@@ -626,16 +646,16 @@ struct Chain {
 #pragma unroll
     for (int t = 0; t < VPL; ++t) {
       if constexpr (RIC) {
-        en[t] = ell_of(r[t], S0 * fmaf(df, s[k][t] - iso[t], M[t]), tau);
+        en[t] = ell_of(yv(t), S0 * fmaf(df, s[k][t] - isov(t), M[t]), tau);
         a += mask(t, en[t] - ell[t]);
-      } else qd.add(r[t], s[k][t] - iso[t]);
+      } else qd.add(r[t], s[k][t] - isov(t));
     }
     a = RIC ? row_sum16(a) : qd.delta(S0 * df);
     if (dlik(a) + (prn - prior<SL>()) < -lu) {
       const float c = S0 * df;
 #pragma unroll
       for (int t = 0; t < VPL; ++t) {
-        const float q = s[k][t] - iso[t];
+        const float q = s[k][t] - isov(t);
         M[t] = fmaf(df, q, M[t]);
         if constexpr (RIC) ell[t] = en[t]; else r[t] = fmaf(-c, q, r[t]);
       }
@@ -666,6 +686,7 @@ struct ChainLds {
   float* iso;
   float* bp;   // --gradnonlin: [16 rows][16 VPL] b values of the rows' voxels
   float* gd;   // --gradnonlin: [16 rows][9] deviation values
+  float* vol;  // [4 waves][VPL][64] per-volume array of the chains (null for --gradnonlin)
 };
 
 // stage the acquisition of `a` (16*VPL volumes, zero padded) into LDS; the
@@ -695,6 +716,7 @@ FDT_DEV void run_chain(const McmcArgs& a, int vox, const ChainLds& L) {
   c.rl = rl;
   c.acq = sacq;
   c.cand = L.cand + (threadIdx.x >> 6) * (K * VPL * 64) + lane;
+  c.vol = L.vol ? L.vol + (threadIdx.x >> 6) * (VPL * 64) + lane : nullptr;
   c.sh = L.sh;
   c.shb = L.shb;
   c.isot = GN ? L.iso : L.iso + (threadIdx.x >> 4) * 16;
@@ -819,12 +841,15 @@ FDT_DEV void run_chain(const McmcArgs& a, int vox, const ChainLds& L) {
     });
     by_kind(ci.kind, [&](auto SK) {
 #pragma unroll
-      for (int t = 0; t < VPL; ++t) { float b = c.bv(t, sacq[rl + 16 * t]); c.iso[t] = sig<decltype(SK)::value>(ci, b, b); }
+      for (int t = 0; t < VPL; ++t) { float b = c.bv(t, sacq[rl + 16 * t]); c.isov(t) = sig<decltype(SK)::value>(ci, b, b); }
     });
   }
   if constexpr (RIC) {
 #pragma unroll
-    for (int t = 0; t < VPL; ++t) c.r[t] = y[t];
+    for (int t = 0; t < VPL; ++t) {
+      if constexpr (C::Y_LDS) c.vol[t * 64] = y[t];
+      else c.r[t] = y[t];
+    }
   }
   c.rebuild(y);
 
@@ -902,7 +927,7 @@ FDT_DEV void run_chain(const McmcArgs& a, int vox, const ChainLds& L) {
       float acc = 0.0f;
 #pragma unroll
       for (int t = 0; t < VPL; ++t) {
-        const float e = c.r[t] - S0 * c.M[t];
+        const float e = c.yv(t) - S0 * c.M[t];
         acc += c.mask(t, e * e);
       }
       ss = (double)row_sum16(acc);

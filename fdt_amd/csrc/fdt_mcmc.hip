@@ -24,7 +24,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VPL <= 18 ?
   __shared__ float siso[GN ? 1 : 16 * 16];
   __shared__ float sbp[GN ? 16 * 16 * VPL : 1];
   __shared__ float sgd[GN ? 16 * 9 : 1];
-  const ChainLds L{sacq, sba, GN ? nullptr : ssh, sshb, siso, sbp, sgd};
+  __shared__ float svol[GN ? 1 : 4 * VPL * 64];
+  const ChainLds L{sacq, sba, GN ? nullptr : ssh, sshb, siso, sbp, sgd, GN ? nullptr : svol};
   stage_acq<VPL>(a, L);
   __syncthreads();
   const int i = xcd_tile(blockIdx.x, gridDim.x) * 16 + (threadIdx.x >> 4);

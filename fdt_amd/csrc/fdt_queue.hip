@@ -26,8 +26,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
   __shared__ int ssh[16 * QUEUE_MAX_VPL];
   __shared__ float sshb[16];
   __shared__ float siso[16 * 16];
+  __shared__ float svol[4 * QUEUE_MAX_VPL * 64];
   __shared__ int sitem, sjob;
-  const ChainLds L{sacq, sba, ssh, sshb, siso};
+  const ChainLds L{sacq, sba, ssh, sshb, siso, nullptr, nullptr, svol};
   const int rows = blockDim.x >> 4;
   const int total = q.first_item[q.njobs];
   for (;;) {
