@@ -170,6 +170,61 @@ FDT_DEV void draw(uint64_t seed, uint64_t vox, uint32_t jump, uint32_t slot, flo
   logu = LN2 * __builtin_amdgcn_logf(u3);
 }
 
+// ------------------------------------------------------------ FP64 exp/log
+// exp and log in FP64 for the LM's model evaluations (the cost of the mixed
+// mode, everything in the FP64 mode), without the special-case handling of
+// libm's (98 and 42 VALU instructions there against ~33 and ~18 here).
+// exp_d: x = n ln2 + r with |r| <= ln2/2 (n ln2 subtracted by two FMAs in the
+// hi/lo split of ln2), Taylor series of e^r to r^13 (truncation below 5e-18),
+// scaled by 2^n (underflows to 0 below -745).  log_d, for positive normal x:
+// x = 2^e m with m in [sqrt(1/2), sqrt(2)), log m = 2 atanh(s) with
+// s = (m-1)/(m+1), |s| <= 0.172, series to s^21 (truncation below 3e-17).
+// Measured over 4M arguments against long double: exp 1.4e-16 and log 3.3e-16
+// largest relative error (libm: 1.4e-16 and 1.3e-16).
+FDT_DEV double exp_d(double x) {
+  const double L2E = 1.4426950408889634, LN2_HI = 0x1.62e42fefa39efp-1, LN2_LO = 0x1.abc9e3b39803fp-56;
+  const double n = __builtin_rint(x * L2E);
+  double r = fma(-n, LN2_HI, x);
+  r = fma(-n, LN2_LO, r);
+  double p = 1.0 / 6227020800.0;
+  p = fma(p, r, 1.0 / 479001600.0);
+  p = fma(p, r, 1.0 / 39916800.0);
+  p = fma(p, r, 1.0 / 3628800.0);
+  p = fma(p, r, 1.0 / 362880.0);
+  p = fma(p, r, 1.0 / 40320.0);
+  p = fma(p, r, 1.0 / 5040.0);
+  p = fma(p, r, 1.0 / 720.0);
+  p = fma(p, r, 1.0 / 120.0);
+  p = fma(p, r, 1.0 / 24.0);
+  p = fma(p, r, 1.0 / 6.0);
+  p = fma(p, r, 0.5);
+  p = fma(p, r, 1.0);
+  p = fma(p, r, 1.0);
+  return __builtin_amdgcn_ldexp(p, (int)n);
+}
+FDT_DEV double log_d(double x) {
+  const double LN2_HI = 0x1.62e42fefa39efp-1, LN2_LO = 0x1.abc9e3b39803fp-56;
+  double m = __builtin_amdgcn_frexp_mant(x);
+  int e = __builtin_amdgcn_frexp_exp(x);
+  if (m < 0.70710678118654752440) { m += m; e -= 1; }
+  const double f = m - 1.0;  // exact
+  const double s = f / (2.0 + f);
+  const double z = s * s;
+  double q = 1.0 / 21.0;
+  q = fma(q, z, 1.0 / 19.0);
+  q = fma(q, z, 1.0 / 17.0);
+  q = fma(q, z, 1.0 / 15.0);
+  q = fma(q, z, 1.0 / 13.0);
+  q = fma(q, z, 1.0 / 11.0);
+  q = fma(q, z, 1.0 / 9.0);
+  q = fma(q, z, 1.0 / 7.0);
+  q = fma(q, z, 1.0 / 5.0);
+  q = fma(q, z, 1.0 / 3.0);
+  const double s2 = s + s;
+  const double ed = (double)e;
+  return fma(ed, LN2_HI, fma(ed, LN2_LO, fma(s2 * z, q, s2)));
+}
+
 // ----------------------------------------------------------- transforms
 template <class T> FDT_DEV T f2x(T f) { return tan(f / T(TWO_PI_D)); }
 template <class T> FDT_DEV T x2f(T x) { return fabs(T(TWO_PI_D) * atan(x)); }

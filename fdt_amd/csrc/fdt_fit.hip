@@ -58,19 +58,19 @@ struct Mdl {
   int G;
   T Rv, invR;
   // exp / log of the model and Jacobian rows: the hardware exp2 / log2 with one
-  // scaling in the FP32 production path (a few ulp, below the FP32 noise of
-  // the fit), the libm functions in the FP64 parity path
+  // scaling in FP32 (a few ulp, below the FP32 noise of the fit); exp_d /
+  // log_d (1-3 ulp) in FP64
   static FDT_DEV T ex(T x) {
     if constexpr (std::is_same<T, float>::value) return __expf(x);
-    else return exp(x);
+    else return exp_d(x);
   }
   static FDT_DEV T rc(T x) {  // 1/x: hardware reciprocal (1 ulp) in FP32
     if constexpr (std::is_same<T, float>::value) return __builtin_amdgcn_rcpf(x);
     else return T(1) / x;
   }
-  static FDT_DEV T lg(T x) {
+  static FDT_DEV T lg(T x) {  // x >= 1 in every model here
     if constexpr (std::is_same<T, float>::value) return __builtin_amdgcn_logf(x) * 0.6931471805599453f;
-    else return log(x);
+    else return log_d(x);
   }
 
   FDT_DEV void prep(const T* p) {
