@@ -487,6 +487,9 @@ static LmArgs base_lm_args(fdt_ctx* c) {
   a.R = c->o.R_prior_mean;
   a.acqv = c->gn ? c->d_acqv.p : nullptr;
   a.acqv_stride = c->acqv_stride;
+  a.shell = c->d_shell.p;
+  a.shell_b = c->d_shell_b.p;
+  a.nshell = c->gn ? 0 : c->nshell;
   return a;
 }
 

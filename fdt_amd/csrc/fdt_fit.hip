@@ -31,6 +31,13 @@ struct Row {
   int N;
   const float4* acq;  // [N]: the workgroup's table in LDS, or the voxel's own (--gradnonlin)
   const float* y;     // LDS, the voxel's signal (sanitised for --rician)
+  // b shells (acquisitions with <= 16 distinct b values, not --gradnonlin):
+  // sh[i] the shell of volume i (LDS), shb[l] its b value, isot the row's
+  // table of the cost model's isotropic signal per shell (LDS); nshell 0 = none
+  int nshell = 0;
+  const int* sh = nullptr;
+  const float* shb = nullptr;
+  double* isot = nullptr;
 };
 
 template <class T>
@@ -150,7 +157,13 @@ struct Mdl {
     }
   }
 
-  FDT_DEV T pred(const T* p, T b, T gx, T gy, T gz) const {
+  // isotropic signal at b (a function of b alone: evaluated once per b shell
+  // where the acquisition has shells)
+  FDT_DEV T iso_val(T b) const {
+    if constexpr (KIND == KIND_MULTI) return ex(-a * lg(T(1) + b * bb));
+    else return ex(-b * d);
+  }
+  FDT_DEV T pred_iso(const T* p, T b, T gx, T gy, T gz, T iso) const {
     T err = T(0);
 #pragma unroll
     for (int k = 0; k < K; ++k) {
@@ -158,16 +171,15 @@ struct Mdl {
       err += fs[k] * aniso(b, dp);
     }
     if constexpr (KIND == KIND_MULTI) {
-      T iso = ex(-a * lg(T(1) + b * bb));
       T S0 = fabs(p[0]);
       if constexpr (F0) return S0 * (tf0 + ((T(1) - sumf - tf0) * iso + err));
       else return S0 * ((T(1) - sumf) * iso + err);
     } else {
-      T iso = ex(-b * d);
       if constexpr (F0) return p[0] * ((tf0 + (T(1) - sumf - tf0) * iso) + err);
       else return p[0] * ((T(1) - sumf) * iso + err);
     }
   }
+  FDT_DEV T pred(const T* p, T b, T gx, T gy, T gz) const { return pred_iso(p, b, gx, gy, gz, iso_val(b)); }
 
   // Jacobian row for the gradient (Jg); the Hessian row equals it except in
   // column 0 for PVM_multi (CUDA/PVM_multi.cu:317 vs :451), returned in J0h.
@@ -304,14 +316,25 @@ struct Mdl {
   }
 };
 
-// cost = sum_i (pred_i - y_i)^2 with FP64 accumulation
+// cost = sum_i (pred_i - y_i)^2 with FP64 accumulation.  With b shells the
+// isotropic signal is evaluated once per shell (lane l: shell l) into the
+// row's table and looked up per volume (the same value as per volume).
 template <class T, class M>
 FDT_DEV double row_cost(const Row& R, const M& m, const T* p) {
   double c = 0.0;
-  for (int i = R.rl; i < R.N; i += 16) {
-    float4 q = R.acq[i];
-    T e = m.pred(p, (T)q.w, (T)q.x, (T)q.y, (T)q.z) - (T)R.y[i];
-    c += (double)e * (double)e;
+  if (R.nshell) {
+    if (R.rl < R.nshell) R.isot[R.rl] = (double)m.iso_val((T)R.shb[R.rl]);
+    for (int i = R.rl; i < R.N; i += 16) {
+      float4 q = R.acq[i];
+      T e = m.pred_iso(p, (T)q.w, (T)q.x, (T)q.y, (T)q.z, (T)R.isot[R.sh[i]]) - (T)R.y[i];
+      c += (double)e * (double)e;
+    }
+  } else {
+    for (int i = R.rl; i < R.N; i += 16) {
+      float4 q = R.acq[i];
+      T e = m.pred(p, (T)q.w, (T)q.x, (T)q.y, (T)q.z) - (T)R.y[i];
+      c += (double)e * (double)e;
+    }
   }
   return row_sum16_d(c);
 }
@@ -984,9 +1007,16 @@ template <class T, class TC, int KIND, int K, bool F0, bool GN>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(std::is_same<T, TC>::value && std::is_same<T, double>::value ? 1 : 2, 2))) void lm_kernel(LmArgs a) {
   extern __shared__ float4 lds4[];
   float4* sacq = lds4;
-  float* sy = reinterpret_cast<float*>(lds4 + a.N);
+  // without shells the table regions are empty (the launcher sizes them)
+  const bool shells = !GN && a.nshell > 0;
+  double* sisot = reinterpret_cast<double*>(lds4 + a.N);
+  float* sy = reinterpret_cast<float*>(sisot + (shells ? LM_ROWS * 16 : 0));
+  int* ssh = reinterpret_cast<int*>(sy + LM_ROWS * ystride(a.N));
   if constexpr (!GN)
-    for (int i = threadIdx.x; i < a.N; i += blockDim.x) sacq[i] = a.acq[i];
+    for (int i = threadIdx.x; i < a.N; i += blockDim.x) {
+      sacq[i] = a.acq[i];
+      if (shells) ssh[i] = a.shell[i];
+    }
   __syncthreads();
   using M = Mdl<T, KIND, K, F0>;
   constexpr int NP = M::NP;
@@ -994,6 +1024,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(std::is_sam
   float* ym = sy + (threadIdx.x >> 4) * ystride(a.N);
   Row R;
   R.rl = rl; R.N = a.N; R.acq = sacq; R.y = ym;
+  if (shells) {
+    R.nshell = a.nshell; R.sh = ssh; R.shb = a.shell_b;
+    R.isot = sisot + (threadIdx.x >> 4) * 16;
+  }
   M m0;
   m0.G = a.G;
   m0.Rv = (T)a.R;
@@ -1231,7 +1265,9 @@ namespace {
 template <int KIND, bool F0, bool GN>
 hipError_t launch_lm(const fdt::LmArgs* a, hipStream_t s) {
   auto kern = fdt::lm_kernel<FitReal, FitCost, KIND, FDT_FIT_K, F0, GN>;
-  size_t lds = (size_t)a->N * sizeof(float4) + (size_t)fdt::LM_ROWS * (16 * (((a->N + 15) / 16) | 1)) * sizeof(float);
+  const bool shells = !GN && a->nshell > 0;
+  size_t lds = (size_t)a->N * sizeof(float4) + (size_t)fdt::LM_ROWS * (16 * (((a->N + 15) / 16) | 1)) * sizeof(float) +
+               (shells ? (size_t)fdt::LM_ROWS * 16 * sizeof(double) + (size_t)a->N * sizeof(int) : 0);
   int dev = 0, cus = 0, per_cu = 0;
   hipError_t e;
   if ((e = hipGetDevice(&dev)) != hipSuccess) return e;

@@ -38,6 +38,9 @@ struct LmArgs {
   int iters_add;        // add to iters instead of overwriting
   const float4* acqv;   // --gradnonlin: per-voxel acquisitions [nvox][acqv_stride], or null
   int acqv_stride;
+  const int* shell;     // [N] b-shell index of each volume, or null
+  const float* shell_b; // [16] b value of each shell
+  int nshell;           // number of shells (0: none; ignored with acqv)
 };
 
 // implausible-f0 test and refit list (CUDA/xfibres_gpu.cu:216-246, 305-339)
