@@ -62,9 +62,12 @@ enum { SL_F0 = 0, SL_TAU = 1, SL_D = 2, SL_DSTD = 3, SL_R = 4, SL_S0 = 5, SL_FIB
 //   kind 0: exp(-d*u)                        model 1
 //   kind 1: (1 + w)^(-d^2/dstd^2), w = (dstd^2/d) u   model 2 (gamma diffusivities);
 //           below d_std 1e-5 the reference switches to exp(-d*u), which the
-//           same expression gives exactly with c0 = c1 = 0 and c2 = -d/ln2,
-//           so rows on either side of the switch share one code path (a
-//           wave whose rows took different forms would run both)
+//           same expression evaluates with c0 = 2^-64, c1 = -d 2^64: then
+//           1 + w rounds to 1, lg2 of it is 0, and the recovered rounding
+//           error is w = c0 u exactly (a power-of-two scaling), so the
+//           exponent is c1 w / ln2 = -d u / ln2 with the roundings of the
+//           exponential form.  Rows on either side of the switch share one
+//           code path (a wave whose rows took different forms would run both).
 //   kind 2: exp(-3 d invR ((1-R) b a + R b))  model 3 zeppelin, u = b a
 // Kind 1 in FP32: u1 = fl(1 + w) keeps w only to 2^-24 absolute, and the
 // exponent d^2/dstd^2 multiplies that error (1e-3 relative signal error at
@@ -75,29 +78,29 @@ enum { SL_F0 = 0, SL_TAU = 1, SL_D = 2, SL_DSTD = 3, SL_R = 4, SL_S0 = 5, SL_FIB
 // ulp of u1, and (d^2/dstd^2) w = d u).
 struct SigC {
   int kind;
-  float c0, c1, c2;
+  float c0, c1;
 };
 FDT_DEV SigC gamma_coef(float d, float dstd) {
   SigC c;
   c.kind = 1;
-  if (dstd < 1e-5f) { c.c0 = 0.0f; c.c1 = 0.0f; c.c2 = -d * L2E; }
-  else { const float s2 = dstd * dstd; c.c0 = s2 / d; c.c1 = -(d * d / s2); c.c2 = 0.0f; }
+  if (dstd < 1e-5f) { c.c0 = 0x1p-64f; c.c1 = -d * 0x1p64f; }
+  else { const float s2 = dstd * dstd; c.c0 = s2 / d; c.c1 = -(d * d / s2); }
   return c;
 }
 FDT_DEV SigC aniso_coef(int model, float d, float dstd, float R) {
   SigC c;
-  if (model == 1) { c.kind = 0; c.c0 = -d * L2E; c.c1 = 0.0f; c.c2 = 0.0f; }
+  if (model == 1) { c.kind = 0; c.c0 = -d * L2E; c.c1 = 0.0f; }
   else if (model == 2) c = gamma_coef(d, dstd);
   else {
     float iR = 1.0f / (2.0f * R + 1.0f);
     float k = -3.0f * d * iR * L2E;
-    c.kind = 2; c.c0 = k * (1.0f - R); c.c1 = k * R; c.c2 = 0.0f;
+    c.kind = 2; c.c0 = k * (1.0f - R); c.c1 = k * R;
   }
   return c;
 }
 FDT_DEV SigC iso_coef(int model, float d, float dstd) {
   SigC c;
-  if (model == 1) { c.kind = 0; c.c0 = -d * L2E; c.c1 = 0.0f; c.c2 = 0.0f; }
+  if (model == 1) { c.kind = 0; c.c0 = -d * L2E; c.c1 = 0.0f; }
   else c = gamma_coef(d, dstd);
   return c;
 }
@@ -107,7 +110,7 @@ FDT_DEV float sig(const SigC& c, float u, float b) {
   else if constexpr (SK == 1) {
     const float u1 = fmaf(c.c0, u, 1.0f);
     const float e = fmaf(c.c0, u, 1.0f - u1);  // (1 + w) - u1 (1 - u1 is exact)
-    return ex2(fmaf(c.c1, fmaf(e, L2E, lg2(u1)), c.c2 * u));
+    return ex2(c.c1 * fmaf(e, L2E, lg2(u1)));
   }
   else return ex2(fmaf(c.c0, u, c.c1 * b));
 }
@@ -198,10 +201,9 @@ struct Chain {
   int scnt;
   // sin and cos of the slot's angle (theta_k / phi_k slots), kept with the
   // value so that fibre directions are row broadcasts, not re-evaluated.
-  // The d and d_std slots hold the stick-signal coefficients of the current
-  // state there instead (c0, c1 in the d lane, c2 in the d_std lane), set
-  // whenever d, d_std or R moves, so that the angle proposals broadcast them
-  // instead of dividing again.
+  // The d slot holds the stick-signal coefficients of the current state
+  // there instead (c0, c1), set whenever d, d_std or R moves, so that the
+  // angle proposals broadcast them instead of dividing again.
   float ssin, scos;
   double like;  // Gaussian: SSR; Rician: sum of ell
   int nv;       // slots of this lane that hold a real volume (rl + 16 t < N)
@@ -261,12 +263,10 @@ struct Chain {
     c.kind = model == 1 ? 0 : (model == 2 ? 1 : 2);
     c.c0 = row_bcast<SL_D>(ssin);
     c.c1 = row_bcast<SL_D>(scos);
-    c.c2 = row_bcast<SL_DSTD>(ssin);
     return c;
   }
   FDT_DEV void set_aniso_cache(const SigC& c) {
     if (rl == SL_D) { ssin = c.c0; scos = c.c1; }
-    if (rl == SL_DSTD) ssin = c.c2;
   }
   // unit directions of the row's fibres (from the theta / phi slots)
   struct Dirs {
@@ -308,11 +308,13 @@ struct Chain {
   FDT_DEV float dlik(float dl) const {
     if constexpr (RIC) return -dl;
     else {
-      // log1p(x) as log(u) * x / (u - 1) with u = 1 + x (exact when u == 1)
-      float x = dl * __builtin_amdgcn_rcpf((float)like);
-      float u = 1.0f + x;
-      float lp = (u == 1.0f) ? x : flog(u) * (x * __builtin_amdgcn_rcpf(u - 1.0f));
-      return halfN * lp;
+      // log1p(x): log of u = fl(1 + x) plus the rounding error of u,
+      // x - (u - 1) (exact for x in [-1/2, 1]; outside, its error is below an
+      // ulp of u against |log u| > 0.4); the dropped factor 1/u leaves an
+      // error below 2^-24 |x| (the same device as the gamma-form signal)
+      const float x = dl * __builtin_amdgcn_rcpf((float)like);
+      const float u = 1.0f + x;
+      return halfN * fmaf(lg2(u), LN2, x - (u - 1.0f));
     }
   }
   // theta prior -log(|sin t| / 2) from st = sin t (fibre.h:283-290)
