@@ -133,9 +133,15 @@ FDT_DEV float logI0f(float x) {
   float Ll = b + flog(pl * __builtin_amdgcn_rsqf(b));
   return b < 3.75f ? Ls : Ll;
 }
-// Rician log-likelihood term of one volume (fibre.h:949-953)
+// Rician log-likelihood term of one volume (fibre.h:949-953),
+//   log y - tau (y^2 + p^2) / 2 + log I0(tau p y),
+// without the part that does not depend on the model signal p: log y and
+// -tau y^2 / 2 cancel from every energy difference except the tau move's,
+// which adds -(tau' - tau) sum y^2 / 2 itself; the energy of a state adds
+// both back (sum log y - tau sum y^2 / 2)
 FDT_DEV float ell_of(float y, float p, float tau) {
-  return flog(y) - 0.5f * tau * (y * y + p * p) + logI0f(tau * p * y);
+  const float tp = tau * p;
+  return fmaf(-0.5f * tp, p, logI0f(tp * y));
 }
 
 // Gaussian likelihood change when every residual moves by -c*q_t:
@@ -403,11 +409,17 @@ struct Chain {
       const float tn = tau + n * width<SL_TAU>();
       if (tn <= 0.0f) { reject<SL_TAU>(); return; }
       const float S0 = v<SL_S0>();
-      float a = 0.0f, en[VPL];
+      float a = 0.0f, yy = 0.0f, en[VPL];
 #pragma unroll
-      for (int t = 0; t < VPL; ++t) { en[t] = ell_of(yv(t), S0 * M[t], tn); a += mask(t, en[t] - ell[t]); }
-      a = row_sum16(a);
-      float dE = -Nf * (flog(tn) - flog(tau)) - a;
+      for (int t = 0; t < VPL; ++t) {
+        en[t] = ell_of(yv(t), S0 * M[t], tn);
+        const float y = yv(t);
+        a += mask(t, en[t] - ell[t]);
+        yy += mask(t, y * y);
+      }
+      a = row_sum16(a);  // change of the stored terms (the running sum's)
+      // the -tau y^2 / 2 part of the full terms moves with tau too
+      float dE = -Nf * (flog(tn) - flog(tau)) - fmaf(-0.5f * (tn - tau), row_sum16(yy), a);
       if (dE < -lu) {
 #pragma unroll
         for (int t = 0; t < VPL; ++t) ell[t] = en[t];
@@ -919,8 +931,19 @@ FDT_DEV void run_chain(const McmcArgs& a, int vox, const ChainLds& L) {
     }
   }
   if (a.energy) {
-    float tau = c.template v<SL_TAU>();
-    if (rl == 0) a.energy[vox] = RIC ? (float)(-(double)a.N * (double)flog(tau) - c.like) : (float)c.like;
+    const float tau = c.template v<SL_TAU>();
+    double e = c.like;
+    if constexpr (RIC) {
+      double sl = 0.0, sy = 0.0;  // the model-independent part of the Rician terms
+#pragma unroll
+      for (int t = 0; t < VPL; ++t) {
+        const double yd = (double)c.yv(t);
+        sl += (double)c.mask(t, flog(c.yv(t)));
+        sy += t < c.nv ? yd * yd : 0.0;
+      }
+      e = -(double)a.N * (double)flog(tau) - (c.like + row_sum16_d(sl) - 0.5 * (double)tau * row_sum16_d(sy));
+    }
+    if (rl == 0) a.energy[vox] = (float)e;
   }
   if (a.ssr) {  // sum of squares of the final state's residuals (the Rician chain keeps the data in r)
     double ss;
