@@ -163,12 +163,6 @@ FDT_DEV void reload_acq() { asm volatile("" ::: "memory"); }
 FDT_DEV int acc1(int c) { return c + 1; }
 FDT_DEV int rej1(int c) { return c + (1 << 16); }
 
-// compile-time loop: f(integral_constant<int, i>) for i = 0..N-1
-template <int N, class F>
-FDT_DEV void sfor(F&& f) {
-  [&]<int... is>(std::integer_sequence<int, is...>) { (f(std::integral_constant<int, is>{}), ...); }(
-      std::make_integer_sequence<int, N>{});
-}
 
 // One chain per 16-lane row.  Volume state lives in registers per lane;
 // everything that belongs to a proposal slot -- the parameter value, its

@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <utility>
 
 #define FDT_DEV __device__ __forceinline__
 
@@ -13,6 +14,13 @@ constexpr double TWO_PI_D = 0.636619772;
 constexpr float FSMALL_F = 0.001f;
 constexpr float UPPERDIFF_F = 0.005f;  // fibre.h:24
 constexpr float MAXPROP_F = 1e10f;     // fibre.h:19
+
+// compile-time loop: f(integral_constant<int, i>) for i = 0..N-1
+template <int N, class F>
+FDT_DEV void sfor(F&& f) {
+  [&]<int... is>(std::integer_sequence<int, is...>) { (f(std::integral_constant<int, is>{}), ...); }(
+      std::make_integer_sequence<int, N>{});
+}
 
 // ----------------------------------------------------------------- lanes
 // All-reduce inside a 16-lane DPP row: quad_perm xor1, quad_perm xor2,

@@ -80,14 +80,31 @@ struct Mdl {
     else return log_d(x);
   }
 
+  template <int L>
+  static FDT_DEV T bc(T v) {
+    if constexpr (std::is_same<T, double>::value) return row_bcast_d<L>(v);
+    else return row_bcast<L>(v);
+  }
+  // The parameters are the same in every lane of the row, so the per-row
+  // transcendentals are evaluated once: lane l takes sin and cos of p[l] (and,
+  // for the x2f fractions, x2f(p[l]) and its derivative), and the row
+  // broadcasts each from its lane, instead of all 16 lanes evaluating all of
+  // them (a double sin or cos is ~150 instructions with its reduction).
   FDT_DEV void prep(const T* p) {
+    const int rl = __lane_id() & 15;
+    // lane l's parameter where l is an angle (theta, phi) or an x2f fraction
+    // slot, 0 elsewhere (no lane takes the slow argument reduction of sin/cos
+    // for S0 or a diffusivity)
+    T pl = T(0);
 #pragma unroll
-    for (int k = 0; k < K; ++k) {
-      int kk = FB + 3 * k;
-      sth[k] = sin(p[kk + 1]); cth[k] = cos(p[kk + 1]);
-      sph[k] = sin(p[kk + 2]); cph[k] = cos(p[kk + 2]);
+    for (int i = FB; i < NP; ++i) if (rl == i) pl = p[i];
+    const T sl = sin(pl), cl = cos(pl);
+    sfor<K>([&](auto kc) {
+      constexpr int k = kc, kk = FB + 3 * k;
+      sth[k] = bc<kk + 1>(sl); cth[k] = bc<kk + 1>(cl);
+      sph[k] = bc<kk + 2>(sl); cph[k] = bc<kk + 2>(cl);
       x0[k] = sth[k] * cph[k]; x1[k] = sth[k] * sph[k]; x2[k] = cth[k];
-    }
+    });
     sumf = T(0);
     if constexpr (KIND == KIND_SINGLE_C) {
       lam = p[1];
@@ -125,25 +142,18 @@ struct Mdl {
 #pragma unroll
       for (int j = 0; j < K; ++j) part -= fs[j];
       if constexpr (F0) tf0 = beta2f<T>(p[NP - 1]) * part;
-    } else if constexpr (KIND == KIND_SINGLE) {
-      d = fabs(p[1]);
-#pragma unroll
-      for (int k = 0; k < K; ++k) { fs[k] = x2f<T>(p[2 + 3 * k]); sumf += fs[k]; }
-      if constexpr (F0) tf0 = x2f<T>(p[NP - 1]);
     } else {
-      a = fabs(p[1]);
-      bb = fabs(p[2]);
-#pragma unroll
-      for (int k = 0; k < K; ++k) { fs[k] = x2f<T>(p[3 + 3 * k]); sumf += fs[k]; }
-      if constexpr (F0) tf0 = x2f<T>(p[NP - 1]);
-    }
-    if constexpr (KIND != KIND_SINGLE_C) {
-#pragma unroll
-      for (int k = 0; k < K; ++k) {
-        const T x = p[FB + 3 * k];
-        dfx[k] = T(TWO_PI_D) * sgn<T>(x) * T(1) / (T(1) + x * x);
-      }
-      if constexpr (F0) dfx0 = T(TWO_PI_D) * sgn<T>(p[NP - 1]) * T(1) / (T(1) + p[NP - 1] * p[NP - 1]);
+      if constexpr (KIND == KIND_SINGLE) d = fabs(p[1]);
+      else { a = fabs(p[1]); bb = fabs(p[2]); }
+      // the fractions f = x2f(x) and d f / d x from the lane of each x
+      const T fl = x2f<T>(pl), dl = T(TWO_PI_D) * sgn<T>(pl) * T(1) / (T(1) + pl * pl);
+      sfor<K>([&](auto kc) {
+        constexpr int k = kc;
+        fs[k] = bc<FB + 3 * k>(fl);
+        dfx[k] = bc<FB + 3 * k>(dl);
+        sumf += fs[k];
+      });
+      if constexpr (F0) { tf0 = bc<NP - 1>(fl); dfx0 = bc<NP - 1>(dl); }
     }
     ip0 = T(1) / p[0];
   }
