@@ -1,9 +1,11 @@
 """Chain edge cases against the oracle: acquisitions whose length leaves
 several padding slots in a lane (N = 50, 100, 200), tiles larger than the
-acquisition needs (bit-identical to the smallest tile), and chains whose
-angles are far outside one revolution (as width adaptation drives the angles
-of unsupported fibres), where sin/cos must be reduced over the full range as
-the reference does (CUDA/runmcmc_kernels.cu:340, :1165, :1177-1182)."""
+acquisition needs (bit-identical to the smallest tile), chains whose angles
+are far outside one revolution (as width adaptation drives the angles of
+unsupported fibres), where sin/cos must be reduced over the full range as
+the reference does (CUDA/runmcmc_kernels.cu:340, :1165, :1177-1182), and an
+acquisition with more distinct b values than a b-shell table holds, where
+the chains and the LM cost evaluate the isotropic signal per volume."""
 import numpy as np
 import pytest
 
@@ -98,4 +100,49 @@ def test_wide_angle_chains_track_fp32_oracle(orc, fdt, model, K):
     np.testing.assert_array_equal(np.sort(g["th"][:, 0, :], axis=0), np.sort(p[:, [3 + 3 * k + add for k in range(K)]].T, axis=0))
     agree = first_sample_agreement(g, o)
     print(f"model {model}: first sample at angles ~1e9 agrees with the FP32 oracle on {agree:.3f} of voxels")
+    assert agree >= 0.75
+
+
+def jittered_b_voxels(nvox=64, seed=5):
+    """c1 voxels acquired with every non-zero b value jittered by U(-30, 30)
+    s/mm^2 (271 distinct values; a b-shell table holds 16), signal and Rician
+    noise (SNR 25) generated for that acquisition."""
+    v = synth.voxels("c1", nvox)
+    g = np.random.default_rng(seed)
+    b = v["bvals"].astype(np.float64).copy()
+    nz = b > 0
+    b[nz] += g.uniform(-30.0, 30.0, int(nz.sum()))
+    b = b.astype(np.float32).astype(np.float64)
+    sig = synth.clean_signal(2, b, v["bvecs"].astype(np.float64), v["S0"], v["d"], v["dstd"], v["f"], v["dirs"])
+    sigma = (v["S0"] / 25.0)[:, None]
+    n1, n2 = g.standard_normal(sig.shape) * sigma, g.standard_normal(sig.shape) * sigma
+    data = np.sqrt((sig + n1) ** 2 + n2 ** 2).astype(np.float32)
+    return dict(bvals=b.astype(np.float32), bvecs=np.ascontiguousarray(v["bvecs"], dtype=np.float32), data=data)
+
+
+def test_many_b_values_lm_and_chains_track_oracle(orc, fdt):
+    from test_gpu_lm import lm_errors
+    v = jittered_b_voxels()
+    assert np.unique(v["bvals"]).size > 16
+    b64, g64, y64 = (v["bvals"].astype(np.float64), v["bvecs"].astype(np.float64), v["data"].astype(np.float64))
+    base = dict(nfibres=3, model=2)
+    want, _, _ = orc.fit(orc.opts(**base), b64, g64, y64)
+    for mode, bar in ((1, 0.8), (2, 0.75)):
+        x = fdt.Xfibres(lm_fp64=mode, **base)
+        try:
+            x.set_acquisition(v["bvals"], v["bvecs"])
+            x.load_data(v["data"])
+            x.fit()
+            got, _ = x.get_params()
+        finally:
+            x.close()
+        e = lm_errors(got.astype(np.float64), want, 2, 3, False)
+        print(f"{np.unique(v['bvals']).size} distinct b values, LM mode {mode}: within 1e-4 {np.mean(e < 1e-4):.3f}")
+        assert np.mean(e < 1e-4) >= bar
+    run = dict(base, nburn=0, njumps=50, sampleevery=25)
+    g = run_engine(fdt, v, want, None, **run)
+    o = orc.mcmc(orc.opts(**run), b64, g64, y64, want.astype(np.float32).astype(np.float64), None)
+    orc.finish(orc.opts(**run), o)
+    agree = first_sample_agreement(g, o)
+    print(f"chains without a b-shell table: first sample agrees on {agree:.3f} of voxels")
     assert agree >= 0.75
