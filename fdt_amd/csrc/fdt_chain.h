@@ -130,8 +130,9 @@ FDT_DEV float logI0f(float x) {
   float Ls = flog(ys);
   float c = 3.75f * __builtin_amdgcn_rcpf(b);
   float pl = 0.39894228f + c * (0.01328592f + c * (0.00225319f + c * (-0.00157565f + c * (0.00916281f + c * (-0.02057706f + c * (0.02635537f + c * (-0.01647633f + c * 0.00392377f)))))));
-  float Ll = b + flog(pl * __builtin_amdgcn_rsqf(b));
-  return b < 3.75f ? Ls : Ll;
+  // one logarithm of the selected branch: log ys, or b + log(pl / sqrt b)
+  const bool sm = b < 3.75f;
+  return flog(sm ? ys : pl * __builtin_amdgcn_rsqf(b)) + (sm ? 0.0f : b);
 }
 // Rician log-likelihood term of one volume (fibre.h:949-953),
 //   log y - tau (y^2 + p^2) / 2 + log I0(tau p y),
