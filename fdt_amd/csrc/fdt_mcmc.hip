@@ -35,10 +35,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VPL <= 18 ?
 }  // namespace fdt
 
 // Two translation units per VPL value (-DFDT_VPL=...): -DFDT_MCMC_RIC=0 holds
-// the Gaussian kernels and the launcher, -DFDT_MCMC_RIC=1 the Rician ones,
-// so each can be compiled with its own instruction scheduler (the Gaussian
-// kernels gain from an ILP-oriented one, under which the Rician kernels spill
-// several times more than under the default).  K = 1..3, with the shared
+// the Gaussian kernels and the launcher, -DFDT_MCMC_RIC=1 the Rician ones
+// (built in parallel; each can take its own compiler options).  K = 1..3, with the shared
 // acquisition or per-voxel ones (--gradnonlin, a->acqv set; its stride must
 // cover 16 * VPL volumes).
 #ifndef FDT_VPL
