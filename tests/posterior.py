@@ -1,5 +1,5 @@
 """Posterior agreement statistics between two finished chains on the same
-voxels (the north-star parity measures): the supported-fibre count (mean f >
+voxels (the posterior parity measures): the supported-fibre count (mean f >
 0.05, the ARD decision), the largest |mean f| difference over fibres, and the
 largest angle between matched posterior-mean dyads on voxels with 2+
 supported fibres on both sides.  Fibres are matched by the permutation of the

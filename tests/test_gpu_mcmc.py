@@ -65,7 +65,7 @@ def test_short_chain_tracks_oracle_draw_for_draw(orc, fdt, model, K):
 @pytest.mark.parametrize("name,model,K,nvox", [("c0", 1, 2, 1024), ("c1", 2, 3, 768)])
 def test_posterior_parity_with_oracle(orc, fdt, name, model, K, nvox):
     """Full-length chains (1,000 burn-in + 1,250 recorded sweeps) on the GPU
-    against the FP64 oracle.  The north-star bar (ARD counts equal on 98% of
+    against the FP64 oracle.  The posterior parity bar (ARD counts equal on 98% of
     voxels, |mean f| within 0.02 and dyads within 2 degrees on 95%) is not met
     by the oracle against itself: with another seed (Monte Carlo noise of 50
     recorded samples), or in the reference's FP32 arithmetic with the same
