@@ -209,6 +209,10 @@ struct Chain {
   double like;  // Gaussian: SSR; Rician: sum of ell
   int nv;       // slots of this lane that hold a real volume (rl + 16 t < N)
   int rl;
+  // slot 0 holds b = 0 volumes only (see stage_acq): the moves that leave
+  // their signals at 1 skip it
+  bool z0;
+  FDT_DEV bool skip(int t) const { return t == 0 && z0; }
   // options
   int model;
   bool incf0, ardf0, can_ard;
@@ -379,6 +383,7 @@ struct Chain {
     Quad qd;
 #pragma unroll
     for (int t = 0; t < VPL; ++t) {
+      if (skip(t)) continue;
       if constexpr (RIC) {
         en[t] = ell_of(yv(t), S0 * fmaf(df, 1.0f - isov(t), M[t]), tau);
         a += mask(t, en[t] - ell[t]);
@@ -389,6 +394,7 @@ struct Chain {
       const float c = S0 * df;
 #pragma unroll
       for (int t = 0; t < VPL; ++t) {
+        if (skip(t)) continue;
         const float q = 1.0f - isov(t);
         M[t] = fmaf(df, q, M[t]);
         if constexpr (RIC) ell[t] = en[t]; else r[t] = fmaf(-c, q, r[t]);
@@ -555,6 +561,7 @@ struct Chain {
         if constexpr (decltype(SH)::value) fill_iso<decltype(SI)::value>(ci);
 #pragma unroll
         for (int t = 0; t < VPL; ++t) {
+          if (skip(t)) continue;
           float mn = mix_new<decltype(SA)::value, decltype(SI)::value, decltype(NA)::value, decltype(NI)::value,
                              decltype(SH)::value>(t, q, ca, ci, D);
           if constexpr (RIC) a += mask(t, ell_of(yv(t), S0 * mn, tau) - ell[t]);
@@ -574,6 +581,7 @@ struct Chain {
         by_shells<decltype(NI)::value>([&](auto SH) {
 #pragma unroll
           for (int t = 0; t < VPL; ++t) {
+            if (skip(t)) continue;
             store_new<decltype(SA)::value, decltype(SI)::value, decltype(NA)::value, decltype(NI)::value,
                       decltype(SH)::value>(t, ca, ci);
             float mn = mix(t, q);
@@ -610,6 +618,7 @@ struct Chain {
     by_kind(ca.kind, [&](auto SK) {
 #pragma unroll
       for (int t = 0; t < VPL; ++t) {
+        if (skip(t)) continue;
         float4 g = acq[rl + 16 * t];
         float dp = fmaf(g.x, x0, fmaf(g.y, x1, g.z * x2));
         sn[t] = sig<decltype(SK)::value>(ca, g.w * dp * dp, bv(t, g));
@@ -618,6 +627,7 @@ struct Chain {
     Quad qd;
 #pragma unroll
     for (int t = 0; t < VPL; ++t) {
+      if (skip(t)) continue;
       if constexpr (RIC) {
         en[t] = ell_of(yv(t), S0 * fmaf(fk, sn[t] - s[k][t], M[t]), tau);
         a += mask(t, en[t] - ell[t]);
@@ -629,6 +639,7 @@ struct Chain {
       const float c = S0 * fk;
 #pragma unroll
       for (int t = 0; t < VPL; ++t) {
+        if (skip(t)) continue;
         const float q = sn[t] - s[k][t];
         M[t] = fmaf(fk, q, M[t]);
         if constexpr (RIC) ell[t] = en[t]; else r[t] = fmaf(-c, q, r[t]);
@@ -654,6 +665,7 @@ struct Chain {
     Quad qd;
 #pragma unroll
     for (int t = 0; t < VPL; ++t) {
+      if (skip(t)) continue;
       if constexpr (RIC) {
         en[t] = ell_of(yv(t), S0 * fmaf(df, s[k][t] - isov(t), M[t]), tau);
         a += mask(t, en[t] - ell[t]);
@@ -664,6 +676,7 @@ struct Chain {
       const float c = S0 * df;
 #pragma unroll
       for (int t = 0; t < VPL; ++t) {
+        if (skip(t)) continue;
         const float q = s[k][t] - isov(t);
         M[t] = fmaf(df, q, M[t]);
         if constexpr (RIC) ell[t] = en[t]; else r[t] = fmaf(-c, q, r[t]);
@@ -696,17 +709,76 @@ struct ChainLds {
   float* bp;   // --gradnonlin: [16 rows][16 VPL] b values of the rows' voxels
   float* gd;   // --gradnonlin: [16 rows][9] deviation values
   float* vol;  // [4 waves][VPL][64] per-volume array of the chains (null for --gradnonlin)
+  int* perm;   // [16 VPL] volume held by each slot position (null for --gradnonlin: identity)
+  int* zc;     // [ZC_CHUNKS] b = 0 volumes per 64-volume chunk of the acquisition
 };
 
+// Volumes with b = 0 have stick and isotropic signals of exactly 1 in every
+// model, so an angle, fraction or f0 move leaves their mixture, residual and
+// likelihood term unchanged, and a diffusivity move changes their mixture by
+// rounding only.  When the acquisition has at least 16 of them, the staging
+// puts them first (a stable partition: b = 0 volumes in order, then the others
+// in order, padding last), so that slot 0 of every row holds b = 0 volumes
+// only, and those proposals skip slot 0.  The likelihood is the same sum over
+// the same volumes; with fewer than 16 the order is the acquisition's.
+constexpr int ZC_CHUNKS = 8;  // 64-volume chunks of the largest table (16 x 32)
+
 // stage the acquisition of `a` (16*VPL volumes, zero padded) into LDS; the
-// caller synchronises the workgroup afterwards
+// caller synchronises the workgroup afterwards.  The block size is a multiple
+// of 64.
 template <int VPL>
 FDT_DEV void stage_acq(const McmcArgs& a, const ChainLds& L) {
-  for (int i = threadIdx.x; i < 16 * VPL; i += blockDim.x) {
-    L.acq[i] = a.acq[i];
-    if (L.sh) L.sh[i] = a.nshell ? a.shell[i] : 0;
+  constexpr int NV = 16 * VPL, NCH = (NV + 63) / 64;
+  static_assert(NCH <= ZC_CHUNKS, "acquisition table larger than the chunk counts");
+  if (!L.perm) {
+    for (int i = threadIdx.x; i < NV; i += blockDim.x) {
+      L.acq[i] = a.acq[i];
+      if (L.sh) L.sh[i] = a.nshell ? a.shell[i] : 0;
+    }
+  } else {
+    // b = 0 count per 64-volume chunk (one wave per chunk), then each volume's
+    // position from the counts of the chunks before it and a ballot
+    const int lane = __lane_id(), w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    for (int ch = w; ch < NCH; ch += nw) {
+      const int i = 64 * ch + lane;
+      bool z = false;
+      if (i < a.N) z = a.acq[i].w == 0.0f;
+      const uint64_t m = __builtin_amdgcn_ballot_w64(z);
+      if (lane == 0) L.zc[ch] = __builtin_popcountll(m);
+    }
+    __syncthreads();
+    int nz = 0;
+#pragma unroll
+    for (int ch = 0; ch < NCH; ++ch) nz += L.zc[ch];
+    const bool part = nz >= 16;
+    for (int ch = w; ch < NCH; ch += nw) {
+      const int i = 64 * ch + lane;
+      const float4 g = i < NV ? a.acq[i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      const bool z = i < a.N && g.w == 0.0f;
+      const uint64_t m = __builtin_amdgcn_ballot_w64(z);
+      int before = __builtin_popcountll(m & ((1ull << lane) - 1ull));  // b = 0 volumes below i
+#pragma unroll
+      for (int c = 0; c < NCH; ++c) before += c < ch ? L.zc[c] : 0;
+      const int pos = (!part || i >= a.N) ? i : (z ? before : nz + i - before);
+      if (i < NV) {
+        L.perm[pos] = i;
+        L.acq[pos] = g;
+        if (L.sh) L.sh[pos] = a.nshell ? a.shell[i] : 0;
+      }
+    }
   }
   if (threadIdx.x < 16) L.shb[threadIdx.x] = a.nshell ? a.shell_b[threadIdx.x] : 0.0f;
+}
+// slot 0 of every row holds b = 0 volumes only (after the caller's
+// synchronisation); uniform over the workgroup
+template <int VPL>
+FDT_DEV bool zero_slot0(const ChainLds& L) {
+  constexpr int NCH = (16 * VPL + 63) / 64;
+  if (!L.perm) return false;
+  int nz = 0;
+#pragma unroll
+  for (int ch = 0; ch < NCH; ++ch) nz += L.zc[ch];
+  return __builtin_amdgcn_readfirstlane(nz) >= 16;
 }
 
 // The whole chain of voxel `vox` on this thread's 16-lane row: initialisation
@@ -756,10 +828,12 @@ FDT_DEV void run_chain(const McmcArgs& a, int vox, const ChainLds& L) {
   // ---- data (with the Rician non-positive replacement, CUDA/xfibres_gpu.cu:350-367)
   float y[VPL];
   const float* yrow = a.data + (size_t)vox * a.N;
+  const int* perm = L.perm;  // volume of each slot position (null: identity)
+  c.z0 = zero_slot0<VPL>(L);
 #pragma unroll
   for (int t = 0; t < VPL; ++t) {
     int i = rl + 16 * t;
-    y[t] = i < a.N ? yrow[i] : 0.0f;
+    y[t] = i < a.N ? yrow[perm ? perm[i] : i] : 0.0f;
   }
   if (RIC) {
     float mn = 3.0e38f, mx = -3.0e38f, mnp = 3.0e38f;
@@ -919,7 +993,7 @@ FDT_DEV void run_chain(const McmcArgs& a, int vox, const ChainLds& L) {
 #pragma unroll
         for (int t = 0; t < VPL; ++t) {
           const int i = rl + 16 * t;
-          yr[t] = i < a.N ? yrow[i] : 0.0f;
+          yr[t] = i < a.N ? yrow[perm ? perm[i] : i] : 0.0f;
         }
         c.rebuild(yr);
       }

@@ -27,8 +27,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
   __shared__ float sshb[16];
   __shared__ float siso[16 * 16];
   __shared__ float svol[4 * QUEUE_MAX_VPL * 64];
+  __shared__ int sperm[16 * QUEUE_MAX_VPL];
+  __shared__ int szc[ZC_CHUNKS];
   __shared__ int sitem, sjob;
-  const ChainLds L{sacq, sba, ssh, sshb, siso, nullptr, nullptr, svol};
+  const ChainLds L{sacq, sba, ssh, sshb, siso, nullptr, nullptr, svol, sperm, szc};
   const int rows = blockDim.x >> 4;
   const int total = q.first_item[q.njobs];
   for (;;) {
