@@ -32,7 +32,7 @@ FP32_PEAK_TFLOPS = 157.3
 # HBM bytes per launch of mcmc_kernel<3,18,false> on the full 250,968-voxel
 # workload: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE, each scaled by the same
 # counter's reading for a copy kernel of known size (profiles/mcmc_c1_traffic.md)
-MCMC_TRAFFIC_BYTES = (2 * 148037 + 617406) * 1024
+MCMC_TRAFFIC_BYTES = (2 * 148124 + 632652) * 1024
 METRIC = "voxels/s, 3-fibre multi-shell bedpostx; surrogate vs plain at equal dyad error"
 LM_MODES = {0: "FP32 model, Jacobian and cost", 1: "FP64 throughout",
             2: "FP32 Jacobian and normal equations, FP64 parameters and cost"}

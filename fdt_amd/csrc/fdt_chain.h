@@ -57,6 +57,12 @@ FDT_DEV void sincos_full(float x, float& s, float& c) {
 // draw slots (fixed per parameter)
 enum { SL_F0 = 0, SL_TAU = 1, SL_D = 2, SL_DSTD = 3, SL_R = 4, SL_S0 = 5, SL_FIB = 6 };
 
+// Tiles whose chains stage b = 0 volumes first and skip their slot (see
+// stage_acq): the 18-volume tile, where it saves 3-4% of the chain time; in
+// the smaller tiles the uniform branch around slot 0 costs more than it saves
+// (it splits the few volumes a lane interleaves).
+template <int VPL> constexpr bool ZERO_SLOT = VPL == 18;
+
 // Signal forms with row-uniform coefficients (compute_signal /
 // compute_iso_signal, CUDA/runmcmc_kernels.cu:67-95):
 //   kind 0: exp(-d*u)                        model 1
@@ -210,9 +216,9 @@ struct Chain {
   int nv;       // slots of this lane that hold a real volume (rl + 16 t < N)
   int rl;
   // slot 0 holds b = 0 volumes only (see stage_acq): the moves that leave
-  // their signals at 1 skip it
+  // their signals at 1 skip it (18-volume tiles only)
   bool z0;
-  FDT_DEV bool skip(int t) const { return t == 0 && z0; }
+  FDT_DEV bool skip(int t) const { return ZERO_SLOT<VPL> && t == 0 && z0; }
   // options
   int model;
   bool incf0, ardf0, can_ard;
@@ -720,7 +726,8 @@ struct ChainLds {
 // puts them first (a stable partition: b = 0 volumes in order, then the others
 // in order, padding last), so that slot 0 of every row holds b = 0 volumes
 // only, and those proposals skip slot 0.  The likelihood is the same sum over
-// the same volumes; with fewer than 16 the order is the acquisition's.
+// the same volumes; with fewer than 16 the order is the acquisition's.  Only
+// the 18-volume tile does this (ZERO_SLOT).
 constexpr int ZC_CHUNKS = 8;  // 64-volume chunks of the largest table (16 x 32)
 
 // stage the acquisition of `a` (16*VPL volumes, zero padded) into LDS; the
@@ -730,7 +737,7 @@ template <int VPL>
 FDT_DEV void stage_acq(const McmcArgs& a, const ChainLds& L) {
   constexpr int NV = 16 * VPL, NCH = (NV + 63) / 64;
   static_assert(NCH <= ZC_CHUNKS, "acquisition table larger than the chunk counts");
-  if (!L.perm) {
+  if (!ZERO_SLOT<VPL> || !L.perm) {
     for (int i = threadIdx.x; i < NV; i += blockDim.x) {
       L.acq[i] = a.acq[i];
       if (L.sh) L.sh[i] = a.nshell ? a.shell[i] : 0;
@@ -774,7 +781,7 @@ FDT_DEV void stage_acq(const McmcArgs& a, const ChainLds& L) {
 template <int VPL>
 FDT_DEV bool zero_slot0(const ChainLds& L) {
   constexpr int NCH = (16 * VPL + 63) / 64;
-  if (!L.perm) return false;
+  if (!ZERO_SLOT<VPL> || !L.perm) return false;
   int nz = 0;
 #pragma unroll
   for (int ch = 0; ch < NCH; ++ch) nz += L.zc[ch];
@@ -828,7 +835,7 @@ FDT_DEV void run_chain(const McmcArgs& a, int vox, const ChainLds& L) {
   // ---- data (with the Rician non-positive replacement, CUDA/xfibres_gpu.cu:350-367)
   float y[VPL];
   const float* yrow = a.data + (size_t)vox * a.N;
-  const int* perm = L.perm;  // volume of each slot position (null: identity)
+  const int* perm = ZERO_SLOT<VPL> ? L.perm : nullptr;  // volume of each slot position (null: identity)
   c.z0 = zero_slot0<VPL>(L);
 #pragma unroll
   for (int t = 0; t < VPL; ++t) {

@@ -25,9 +25,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VPL <= 18 ?
   __shared__ float sbp[GN ? 16 * 16 * VPL : 1];
   __shared__ float sgd[GN ? 16 * 9 : 1];
   __shared__ float svol[GN ? 1 : 4 * VPL * 64];
-  __shared__ int sperm[GN ? 1 : 16 * VPL];
-  __shared__ int szc[ZC_CHUNKS];
-  const ChainLds L{sacq, sba, GN ? nullptr : ssh, sshb, siso, sbp, sgd, GN ? nullptr : svol, GN ? nullptr : sperm, szc};
+  constexpr bool ZS = ZERO_SLOT<VPL> && !GN;
+  __shared__ int sperm[ZS ? 16 * VPL : 1];
+  __shared__ int szc[ZS ? ZC_CHUNKS : 1];
+  const ChainLds L{sacq, sba, GN ? nullptr : ssh, sshb, siso, sbp, sgd, GN ? nullptr : svol, ZS ? sperm : nullptr,
+                   ZS ? szc : nullptr};
   stage_acq<VPL>(a, L);
   __syncthreads();
   const int i = xcd_tile(blockIdx.x, gridDim.x) * 16 + (threadIdx.x >> 4);
