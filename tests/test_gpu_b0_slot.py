@@ -43,9 +43,15 @@ def chains(fdt, a, params, tau, **run):
         x.close()
 
 
-@pytest.mark.parametrize("model,rician", [(2, 0), (3, 1)])
-def test_interleaved_b0_volumes_give_the_b0_first_chains(fdt, model, rician):
-    v = synth.voxels("c1", 96)
+def first_volumes(v, n):
+    return dict(bvals=v["bvals"][:n].copy(), bvecs=np.ascontiguousarray(v["bvecs"][:, :n]),
+                data=np.ascontiguousarray(v["data"][:, :n]))
+
+
+# N = 288 fills the 18-volume tile; N = 200 leaves 88 padding slots after the partition
+@pytest.mark.parametrize("model,rician,N", [(2, 0, 288), (3, 1, 288), (2, 0, 200)])
+def test_interleaved_b0_volumes_give_the_b0_first_chains(fdt, model, rician, N):
+    v = first_volumes(synth.voxels("c1", 96), N)
     assert int((v["bvals"] == 0).sum()) == 18 and (v["bvals"][:18] == 0).all()
     w = interleave(v)
     run = dict(nfibres=3, model=model, rician=rician, nburn=100, njumps=250, sampleevery=25)
