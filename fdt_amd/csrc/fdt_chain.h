@@ -58,9 +58,11 @@ FDT_DEV void sincos_full(float x, float& s, float& c) {
 enum { SL_F0 = 0, SL_TAU = 1, SL_D = 2, SL_DSTD = 3, SL_R = 4, SL_S0 = 5, SL_FIB = 6 };
 
 // Tiles whose chains stage b = 0 volumes first and skip their slot (see
-// stage_acq): the 18-volume tile, where it saves 3-4% of the chain time; in
+// stage_acq): the 18-volume tile, where it saves 3-4% of the chain time.  In
 // the smaller tiles the uniform branch around slot 0 costs more than it saves
-// (it splits the few volumes a lane interleaves).
+// (it splits the few volumes a lane interleaves); at 24 and 32 volumes per
+// lane (one wave per SIMD) it saved nothing (N = 384 with 24 b = 0 volumes)
+// and cost 1.4% (N = 432).
 template <int VPL> constexpr bool ZERO_SLOT = VPL == 18;
 
 // Signal forms with row-uniform coefficients (compute_signal /
