@@ -72,13 +72,17 @@ def test_interleaved_b0_volumes_give_the_b0_first_chains(fdt, model, rician, N):
     np.testing.assert_array_equal(e_got, e_want)
 
 
-def test_queue_stages_interleaved_b0_volumes_like_the_per_volume_launch(fdt):
+# one wave (4 voxels) or four waves (16 voxels) per workgroup: the staging runs on whole waves
+@pytest.mark.parametrize("rows_per_wg", [4, 16])
+def test_queue_stages_interleaved_b0_volumes_like_the_per_volume_launch(fdt, rows_per_wg):
     w = interleave(synth.voxels("c2_288", 45))
     kw = dict(nfibres=3, model=2, nburn=40, njumps=100, sampleevery=25)
     q = fdt.Queue()
     try:
         q.enqueue(w["bvals"], w["bvecs"], w["data"], vox_offset=0, **kw)
+        q.set_tiles(rows_per_wg=rows_per_wg, job=0, vpl=18)
         q.run()
+        assert q.timing()["rows_per_wg"] == rows_per_wg
         got = q.outputs(0)
         want = fdt.run_xfibres(w["bvals"], w["bvecs"], w["data"], vox_offset=0, **kw)
         for k in ("th", "ph", "f", "mean_S0", "mean_d", "dyads"):
