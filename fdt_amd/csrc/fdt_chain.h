@@ -717,7 +717,7 @@ struct ChainLds {
   float* bp;   // --gradnonlin: [16 rows][16 VPL] b values of the rows' voxels
   float* gd;   // --gradnonlin: [16 rows][9] deviation values
   float* vol;  // [4 waves][VPL][64] per-volume array of the chains (null for --gradnonlin)
-  int* perm;   // [16 VPL] volume held by each slot position (null for --gradnonlin: identity)
+  int* perm;   // [16 VPL] volume held by each slot position (null: identity)
   int* zc;     // [ZC_CHUNKS] b = 0 volumes per 64-volume chunk of the acquisition
 };
 
@@ -729,7 +729,9 @@ struct ChainLds {
 // in order, padding last), so that slot 0 of every row holds b = 0 volumes
 // only, and those proposals skip slot 0.  The likelihood is the same sum over
 // the same volumes; with fewer than 16 the order is the acquisition's.  Only
-// the 18-volume tile does this (ZERO_SLOT).
+// the 18-volume tile does this (ZERO_SLOT).  With --gradnonlin the per-voxel
+// b values are b |(I+L) g|^2, zero exactly where the shared table's are, and
+// the rows read them in the staged order.
 constexpr int ZC_CHUNKS = 8;  // 64-volume chunks of the largest table (16 x 32)
 
 // stage the acquisition of `a` (16*VPL volumes, zero padded) into LDS; the
@@ -816,8 +818,12 @@ FDT_DEV void run_chain(const McmcArgs& a, int vox, const ChainLds& L) {
     float* bp = L.bp + (threadIdx.x >> 4) * (16 * VPL);
     float* gd = L.gd + (threadIdx.x >> 4) * 9;
     const float4* av = a.acqv + (size_t)vox * a.acqv_stride;
+    const int* pm = ZERO_SLOT<VPL> ? L.perm : nullptr;  // the staged order of the shared table
 #pragma unroll
-    for (int t = 0; t < VPL; ++t) bp[rl + 16 * t] = av[rl + 16 * t].w;
+    for (int t = 0; t < VPL; ++t) {
+      const int i = rl + 16 * t;
+      bp[i] = av[pm ? pm[i] : i].w;
+    }
     if (rl < 9) gd[rl] = a.gdev[(size_t)vox * 9 + rl];
     c.bp = bp;
     c.gd = gd;

@@ -25,7 +25,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VPL <= 18 ?
   __shared__ float sbp[GN ? 16 * 16 * VPL : 1];
   __shared__ float sgd[GN ? 16 * 9 : 1];
   __shared__ float svol[GN ? 1 : 4 * VPL * 64];
-  constexpr bool ZS = ZERO_SLOT<VPL> && !GN;
+  constexpr bool ZS = ZERO_SLOT<VPL>;
   __shared__ int sperm[ZS ? 16 * VPL : 1];
   __shared__ int szc[ZS ? ZC_CHUNKS : 1];
   const ChainLds L{sacq, sba, GN ? nullptr : ssh, sshb, siso, sbp, sgd, GN ? nullptr : svol, ZS ? sperm : nullptr,

@@ -5,8 +5,8 @@ already put their b = 0 volumes first, so the partition is the identity there;
 here the 18 b = 0 volumes of the multi-shell acquisition are spread through the
 diffusion-weighted ones (relative orders kept).  Both orders stage the same
 table, so the interleaved acquisition must give exactly the chains of the
-b = 0-first one, Gaussian and Rician, and the batched queue must stage it as
-the per-volume launch does."""
+b = 0-first one, Gaussian, Rician and --gradnonlin, and the batched queue must
+stage it as the per-volume launch does."""
 import numpy as np
 import pytest
 
@@ -29,11 +29,13 @@ def interleave(v):
     return dict(bvals=b[P], bvecs=np.ascontiguousarray(v["bvecs"][:, P]), data=np.ascontiguousarray(v["data"][:, P]))
 
 
-def chains(fdt, a, params, tau, **run):
+def chains(fdt, a, params, tau, grad=None, **run):
     x = fdt.Xfibres(**run)
     try:
         x.set_acquisition(a["bvals"], a["bvecs"])
         x.load_data(a["data"])
+        if grad is not None:
+            x.load_grad_dev(grad)
         x.set_params(params, tau)
         x.mcmc(0)
         e = x.energy()
@@ -89,3 +91,27 @@ def test_queue_stages_interleaved_b0_volumes_like_the_per_volume_launch(fdt, row
             np.testing.assert_array_equal(got[k], want[k], err_msg=k)
     finally:
         q.close()
+
+
+def test_interleaved_b0_volumes_gradnonlin(fdt):
+    """--gradnonlin: the per-voxel b values are zero exactly where the shared
+    table's are, and the rows read them in the staged order"""
+    v = synth.voxels("c1", 64)
+    w = interleave(v)
+    grad = synth.grad_dev(np.random.default_rng(5), 64)
+    run = dict(nfibres=3, model=2, nburn=100, njumps=250, sampleevery=25)
+    x = fdt.Xfibres(**run)
+    try:
+        x.set_acquisition(v["bvals"], v["bvecs"])
+        x.load_data(v["data"])
+        x.load_grad_dev(grad)
+        x.fit()
+        params, _ = x.get_params()
+    finally:
+        x.close()
+    want, e_want = chains(fdt, v, params, None, grad=grad, **run)
+    got, e_got = chains(fdt, w, params, None, grad=grad, **run)
+    assert np.isfinite(want["th"]).all()
+    for k in want:
+        np.testing.assert_array_equal(got[k], want[k], err_msg=k)
+    np.testing.assert_array_equal(e_got, e_want)
