@@ -10,6 +10,7 @@ from __future__ import annotations
 import ctypes as C
 import os
 import subprocess
+import weakref
 
 import numpy as np
 
@@ -158,6 +159,42 @@ def _f32(a):
 
 class FdtError(RuntimeError):
     pass
+
+
+_hip_rt = None
+
+
+def _hip():
+    """the HIP runtime the library is linked against (for page-locked host memory)"""
+    global _hip_rt
+    if _hip_rt is None:
+        lib()
+        h = C.CDLL("libamdhip64.so")
+        h.hipHostMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t, C.c_uint]
+        h.hipHostMalloc.restype = C.c_int
+        h.hipHostFree.argtypes = [C.c_void_p]
+        h.hipHostFree.restype = C.c_int
+        _hip_rt = h
+    return _hip_rt
+
+
+def pinned_empty(shape, dtype=np.float32):
+    """A numpy array in page-locked host memory (hipHostMalloc): load_data from
+    it and outputs(into=...) into it copy at the DMA rate instead of through the
+    runtime's staging of pageable memory.  Freed when the array and its views
+    are gone (memory still held at interpreter exit is left to the process end)."""
+    dt = np.dtype(dtype)
+    count = int(np.prod(shape))
+    nbytes = max(count * dt.itemsize, 1)
+    h = _hip()
+    p = C.c_void_p()
+    rc = h.hipHostMalloc(C.byref(p), nbytes, 0)
+    if rc != 0 or not p.value:
+        raise FdtError(f"hipHostMalloc of {nbytes} bytes failed (error {rc})")
+    buf = (C.c_char * nbytes).from_address(p.value)
+    fin = weakref.finalize(buf, h.hipHostFree, p.value)
+    fin.atexit = False
+    return np.frombuffer(buf, dtype=dt, count=count).reshape(shape)
 
 
 class Xfibres:
@@ -336,38 +373,60 @@ class Xfibres:
     def surrogate_start(self, surrogate):
         surrogate._chk(self.L.fdt_surrogate_infer(surrogate.h, self.h))
 
-    def outputs(self, raw_scalars=True):
-        arr, out = _alloc_outputs(self.o, self.nvox, raw_scalars)
+    def output_buffers(self, raw_scalars=True, alloc=None):
+        """arrays of the shapes outputs() fills, from alloc(shape) (e.g.
+        pinned_empty; default zero-filled numpy), to pass as outputs(into=...)"""
+        return _alloc_outputs(self.o, self.nvox, raw_scalars, alloc)[0]
+
+    def outputs(self, raw_scalars=True, into=None):
+        """every output of the last finished run; into: arrays from
+        output_buffers() (same options and voxel count) filled in place"""
+        if into is None:
+            arr, out = _alloc_outputs(self.o, self.nvox, raw_scalars)
+        else:
+            want = _alloc_outputs(self.o, self.nvox, raw_scalars, shapes_only=True)
+            for k, shape in want.items():
+                a = into.get(k)
+                if a is None or a.shape != shape or a.dtype != np.float32 or not a.flags.c_contiguous:
+                    raise ValueError(f"outputs(into=...): {k} must be a C-contiguous float32 array of shape {shape}")
+            arr = into
+            out = Outputs()
+            for n in _OUT_FIELDS:
+                setattr(out, n, arr[n].ctypes.data_as(_FP) if n in want else _FP())
         self._chk(self.L.fdt_get_outputs(self.h, C.byref(out)))
         return arr
 
 
-def _alloc_outputs(o, nv, raw_scalars=True):
-    """numpy buffers for every output the options produce, and the fdt_outputs view of them"""
+def _alloc_outputs(o, nv, raw_scalars=True, alloc=None, shapes_only=False):
+    """buffers for every output the options produce (alloc(shape), default
+    zero-filled numpy), and the fdt_outputs view of them; shapes_only: the
+    shapes alone"""
     K, ns = o.nfibres, o.njumps // o.sampleevery
-    arr = dict(th=np.zeros((K, ns, nv), np.float32), ph=np.zeros((K, ns, nv), np.float32),
-               f=np.zeros((K, ns, nv), np.float32), mean_S0=np.zeros(nv, np.float32),
-               mean_d=np.zeros(nv, np.float32), mean_f=np.zeros((K, nv), np.float32),
-               dyads=np.zeros((K, 3, nv), np.float32), disp=np.zeros((K, nv), np.float32))
+    shapes = dict(th=(K, ns, nv), ph=(K, ns, nv), f=(K, ns, nv), mean_S0=(nv,), mean_d=(nv,), mean_f=(K, nv),
+                  dyads=(K, 3, nv), disp=(K, nv))
     if o.model >= 2:
-        arr["mean_dstd"] = np.zeros(nv, np.float32)
+        shapes["mean_dstd"] = (nv,)
     if o.model == 3:
-        arr["mean_R"] = np.zeros(nv, np.float32)
+        shapes["mean_R"] = (nv,)
     if o.f0:
-        arr["mean_f0"] = np.zeros(nv, np.float32)
+        shapes["mean_f0"] = (nv,)
     if o.rician:
-        arr["mean_tau"] = np.zeros(nv, np.float32)
+        shapes["mean_tau"] = (nv,)
     if raw_scalars:
-        arr["S0"] = np.zeros((ns, nv), np.float32)
-        arr["d"] = np.zeros((ns, nv), np.float32)
+        shapes["S0"] = (ns, nv)
+        shapes["d"] = (ns, nv)
         if o.model >= 2:
-            arr["dstd"] = np.zeros((ns, nv), np.float32)
+            shapes["dstd"] = (ns, nv)
         if o.model == 3:
-            arr["R"] = np.zeros((ns, nv), np.float32)
+            shapes["R"] = (ns, nv)
         if o.f0:
-            arr["f0"] = np.zeros((ns, nv), np.float32)
+            shapes["f0"] = (ns, nv)
         if o.rician:
-            arr["tau"] = np.zeros((ns, nv), np.float32)
+            shapes["tau"] = (ns, nv)
+    if shapes_only:
+        return shapes
+    make = alloc or (lambda shape: np.zeros(shape, np.float32))
+    arr = {k: make(shape) for k, shape in shapes.items()}
     out = Outputs()
     for n in _OUT_FIELDS:
         setattr(out, n, arr[n].ctypes.data_as(_FP) if n in arr else _FP())
