@@ -169,7 +169,15 @@ def _hip():
     global _hip_rt
     if _hip_rt is None:
         lib()
-        h = C.CDLL("libamdhip64.so")
+        h = None
+        for name in ("libamdhip64.so", "/opt/rocm/lib/libamdhip64.so"):
+            try:
+                h = C.CDLL(name)
+                break
+            except OSError:
+                continue
+        if h is None:
+            raise FdtError("the HIP runtime (libamdhip64.so) is not loadable")
         h.hipHostMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t, C.c_uint]
         h.hipHostMalloc.restype = C.c_int
         h.hipHostFree.argtypes = [C.c_void_p]
